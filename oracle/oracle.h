@@ -1,0 +1,96 @@
+/*
+ * oracle.h -- CPU restatement of the reference's Frank-Wolfe UE path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the HIP product is
+ * compared against; it is linked or loaded only by tests/, by the smoke check
+ * and by the CPU-baseline leg of bench.py.  Nothing in the product path
+ * (traffic_assignment_amd/, include/) may call it.
+ *
+ * Every function restates the reference loop for loop (file:line given at
+ * each function), in double precision with no FMA contraction and no
+ * reassociation, so that its results are bit-identical to the reference's
+ * when both are built with -O3 -fno-associative-math -ffp-contract=off.
+ *
+ * Node ids are 1-based as in the reference (index 0 unused); arrays indexed
+ * by node have num_nodes+1 entries, Va has num_nodes+2.
+ *
+ * Parity pins (checked in tests/):
+ *   ora_sssp_pape_lll, ora_sssp_pape   -- golden dist/prevlink from the
+ *       compiled reference, Floyd-Warshall/brute force on tiny graphs, scipy
+ *       Dijkstra on the reference's rmat.txt.* graphs, fixed-point invariants.
+ *   ora_bpr, ora_integral, ora_line_search -- closed forms, hand cases,
+ *       golden lambda sequences.
+ *   ora_fw_assign -- brute-force path sums; golden flows.
+ *   ora_fw_solve  -- golden per-iteration scalars and final flows from the
+ *       compiled reference (tap_frankwolfe_pthread.c built from source).
+ */
+#ifndef TAP_ORACLE_H
+#define TAP_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int64_t num_zones, num_nodes, first_thru_node, num_links;
+  const int64_t *init, *term;               /* [E], sorted by (init, term) */
+  const double *capacity, *fftime, *B, *power;  /* [E] */
+} ora_net;
+
+typedef struct {
+  int64_t num_zones;
+  const int64_t *start;    /* [Z+1]: origin o's items are [start[o-1], start[o]) */
+  const int64_t *dest;     /* 1-based zone ids */
+  const double *demand;
+} ora_demand;
+
+/* per-iteration record of the FW loop (one entry per completed iteration) */
+typedef struct {
+  double lambda, gap, objective, lowerbound, best_lowerbound, relgap;
+  double avg_excess, vht;  /* vht = total_link_cost at the pre-step volumes */
+} ora_iter;
+
+void ora_build_csr(const ora_net *net, int64_t *Va, int64_t *Ea, double *Wa);
+
+void ora_sssp_pape_lll(const int64_t *Va, const int64_t *Ea, const double *Wa,
+                       int64_t num_nodes, int64_t start_node, int64_t first_thru_node,
+                       int64_t *prevlink, double *dist, int64_t *queue_next);
+void ora_sssp_pape(const int64_t *Va, const int64_t *Ea, const double *Wa,
+                   int64_t num_nodes, int64_t start_node, int64_t first_thru_node,
+                   int64_t *prevlink, double *dist, int64_t *queue_next);
+
+/* all origins 1..Z: dist/pred in the node-major [v*Z + s] layout (s 1-based) */
+void ora_sssp_all(const int64_t *Va, const int64_t *Ea, const double *Wa,
+                  int64_t num_nodes, int64_t num_zones, int64_t first_thru_node,
+                  int lll, double *dist, int64_t *pred, int nthreads);
+
+double ora_bpr(double t0, double cap, double B, double power, double Q);
+double ora_integral(double t0, double cap, double B, double power, double Q);
+double ora_dirderiv(const ora_net *net, const double *cap, const double *x,
+                    const double *d, double lambda);
+double ora_objective(const ora_net *net, const double *cap, const double *x);
+double ora_total_cost(int64_t E, const double *x, const double *c);
+double ora_line_search(const ora_net *net, const double *cap, const double *x,
+                       const double *d);
+void ora_update_costs(const ora_net *net, const double *cap, const double *x, double *c);
+double ora_demand_sum(const ora_demand *dem);
+
+/* AON loading with Pape-LLL trees; returns 0, or -1 (and *bad_orig/dest) on
+   a positive demand with no path. nthreads<=1: serial. */
+int ora_fw_assign(const ora_net *net, const ora_demand *dem, const int64_t *Va,
+                  const int64_t *Ea, const double *Wa, double *y, int nthreads,
+                  int64_t *bad_orig, int64_t *bad_dest);
+
+/* Frank-Wolfe solve.  x_init != NULL: warm start from those volumes.
+   Writes final volumes/costs, returns 0 or -1 (no path) / -2 (bad input). */
+int ora_fw_solve(const ora_net *net, const ora_demand *dem, int max_iters,
+                 double target_relgap, int nthreads, const double *x_init,
+                 double *x_out, double *c_out, ora_iter *trace, int trace_cap,
+                 int *iters_out, double *relgap_out, double *obj_out, double *vht_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
