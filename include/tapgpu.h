@@ -1,0 +1,165 @@
+/*
+ * tapgpu.h -- C ABI of the MI355X (gfx950) Frank-Wolfe user-equilibrium
+ * engine.  Plain pointers and sizes only; every function returns a status
+ * code (TAPGPU_OK == 0) and never exits the process.  Host pointers unless
+ * stated otherwise; outputs are written before the call returns.
+ *
+ * Data model (the reference's, trunk/tap/tap_types.h:20-54): node ids are
+ * 1-based, zones are nodes 1..num_zones, nodes below first_thru_node are
+ * zones that are labelled but never expanded (trunk/tap/sssp_pape.c:355),
+ * links are sorted by (init, term) so that link index == CSR edge index
+ * (trunk/tap/tap_functions.c:313-339).  OD demand is an origin-major CSR:
+ * origin o's items are [od_start[o-1], od_start[o]); items with dest == o
+ * or demand <= 0 are ignored (trunk/tap/tap_frankwolfe.c:146-147).
+ *
+ * One context = one network topology holding num_variants >= 1 independent
+ * TAP instances (the batched upgrade-subset form); a plain solve is
+ * num_variants == 1.  A context is used from one host thread at a time.
+ */
+#ifndef TAPGPU_H
+#define TAPGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  TAPGPU_OK = 0,
+  TAPGPU_E_ARG = -1,      /* invalid argument / malformed network            */
+  TAPGPU_E_HIP = -2,      /* HIP runtime error (message in tapgpu_last_error) */
+  TAPGPU_E_NOPATH = -3,   /* positive demand between disconnected zones      */
+  TAPGPU_E_NOMEM = -4,    /* device allocation failed                        */
+  TAPGPU_E_STATE = -5,    /* call out of order (e.g. run before init)        */
+  TAPGPU_E_LIMIT = -6     /* a size or sweep limit was exceeded              */
+};
+
+typedef struct tapgpu_ctx tapgpu_ctx;
+
+/* network (one topology); per-link arrays have num_links entries */
+typedef struct {
+  int64_t num_zones, num_nodes, first_thru_node, num_links;
+  const int64_t *init_node, *term_node;        /* 1-based, sorted (init, term) */
+  const double *capacity, *free_flow_time, *B, *power;
+} tapgpu_network;
+
+typedef struct {
+  int64_t num_zones;
+  const int64_t *od_start;   /* [num_zones + 1] */
+  const int64_t *od_dest;    /* 1-based zone ids */
+  const double *od_demand;
+} tapgpu_demand;
+
+/* per-iteration record of a Frank-Wolfe run (one per completed iteration) */
+typedef struct {
+  double lambda, gap, objective, lowerbound, best_lowerbound, relgap;
+  double avg_excess, vht;    /* vht: total link cost at the pre-step volumes */
+} tapgpu_iter;
+
+typedef struct {
+  int32_t iterations;        /* iterations completed in total          */
+  int32_t converged;         /* relgap <= target reached               */
+  double relgap, objective;  /* at the last completed iteration        */
+  double vht;                /* total link cost after the last step    */
+} tapgpu_result;
+
+/* --- context ------------------------------------------------------------- */
+
+/* Replaces the per-call setup of pape_cuda (trunk/tap/pape_cuda_host.cu:51-95)
+   and link_volume_data_setup (trunk/tap/volume_update_host.h:22-24): the
+   graph, link parameters and demand are uploaded once and stay in HBM.
+   Capacities <= 0 are replaced by the smallest positive capacity
+   (trunk/tap/tap_frankwolfe_pthread.c:654-669). */
+int tapgpu_create(tapgpu_ctx **ctx, const tapgpu_network *net, const tapgpu_demand *dem);
+
+/* Batched form: num_variants instances over the union topology `net`.
+   present[v*num_links + k] (NULL = all present) says whether link k exists
+   in variant v; capacity_v / free_flow_time_v / B_v / power_v (each NULL =
+   the base value) are [num_variants * num_links] per-variant link records
+   (ADD / CHANGE modifications, trunk/tap/tap_mpi_main.c:201-240).  All
+   variants share the demand. */
+int tapgpu_create_batched(tapgpu_ctx **ctx, const tapgpu_network *net,
+                          const tapgpu_demand *dem, int64_t num_variants,
+                          const uint8_t *present, const double *capacity_v,
+                          const double *free_flow_time_v, const double *B_v,
+                          const double *power_v);
+
+void tapgpu_destroy(tapgpu_ctx *ctx);
+
+/* last error message of this context (or of the last failed create) */
+const char *tapgpu_last_error(const tapgpu_ctx *ctx);
+
+int64_t tapgpu_num_variants(const tapgpu_ctx *ctx);
+
+/* --- shortest paths -------------------------------------------------------- */
+
+/* Set the link costs Wa (variant v, link order).  Replaces the per-call Wa
+   upload of pape_cuda (trunk/tap/pape_cuda_host.cu:88-91). */
+int tapgpu_set_costs(tapgpu_ctx *ctx, int64_t variant, const double *costs);
+
+/* All-origin label-correcting SSSP for variant v from the given start nodes
+   (1-based), replacing pape_cuda (trunk/tap/pape_cuda_host.h:15-19).
+   dist / pred (either may be NULL) receive node-major results:
+   dist[node * ld + col_of(i)] for node 0..num_nodes, where col_of(i) =
+   col_base + i for start node i.  With ld = num_start and col_base = 0
+   this is a dense [(num_nodes+1) x num_start] matrix; pape_cuda's own
+   layout is ld = num_start, col_base = 1 (column index = start node id when
+   start nodes are 1..num_start).  dist is DBL_MAX where unreachable; pred is
+   the link index of the last link of the path (-1: none).  Distances are
+   bit-identical to the reference's CPU d'Esopo-Pape; pred matches wherever
+   the shortest path is unique. */
+int tapgpu_sssp(tapgpu_ctx *ctx, int64_t variant, const int64_t *start_nodes,
+                int64_t num_start, double *dist, int64_t *pred, int64_t ld,
+                int64_t col_base);
+
+/* All-or-nothing loading with the current costs of variant v into
+   volumes[num_links] (overwritten), replacing fw_assign
+   (trunk/tap/tap_frankwolfe_cuda.h:26-29) and link_volume_update
+   (trunk/tap/volume_update_host.h:18-20).  Returns TAPGPU_E_NOPATH (with
+   the origin/dest in the error message) where the reference exits. */
+int tapgpu_aon(tapgpu_ctx *ctx, int64_t variant, double *volumes);
+
+/* --- Frank-Wolfe ----------------------------------------------------------- */
+
+/* Initial solution for every variant: all-or-nothing at free-flow costs
+   (warm_volumes == NULL) or the given volumes [num_variants * num_links]
+   (warm start, trunk/tap/tap_frankwolfe_pthread.c:549-604,682-690); costs
+   are then BPR(volumes).  Resets iteration counters and the trace. */
+int tapgpu_fw_init(tapgpu_ctx *ctx, const double *warm_volumes);
+
+/* Run Frank-Wolfe iterations (the loop of tap_frankwolfe_pthread.c:712-793)
+   until each variant has done max_iters iterations in total (<= 0: no cap)
+   or its relative gap is <= target_relgap (<= 0: no target), whichever
+   comes first.  Every variant stops independently; its state is then
+   frozen.  results (may be NULL) receives num_variants records. */
+int tapgpu_fw_run(tapgpu_ctx *ctx, int max_iters, double target_relgap,
+                  tapgpu_result *results);
+
+/* Volumes and costs of variant v (link order of the context's network). */
+int tapgpu_get_flows(tapgpu_ctx *ctx, int64_t variant, double *volumes, double *costs);
+
+/* Copy up to cap per-iteration records of variant v; *count = number held. */
+int tapgpu_get_trace(tapgpu_ctx *ctx, int64_t variant, tapgpu_iter *trace, int cap,
+                     int *count);
+
+/* Device buffers of the current state (for zero-copy consumers such as the
+   torch extension): volumes / costs are [num_variants * num_links] FP64 in
+   HBM, valid until the next call on the context. */
+int tapgpu_device_state(tapgpu_ctx *ctx, const double **d_volumes, const double **d_costs);
+
+/* --- instrumentation ------------------------------------------------------- */
+
+typedef struct {
+  double sssp_ms, aon_ms, link_ms, total_ms;   /* wall time of the last fw_run */
+  int64_t sssp_sweeps, aon_levels;             /* launches in the last fw_run  */
+  int64_t sssp_items, aon_items;               /* (node, origin-group) visits  */
+  int64_t link_passes;
+} tapgpu_stats;
+
+int tapgpu_get_stats(tapgpu_ctx *ctx, tapgpu_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TAPGPU_H */

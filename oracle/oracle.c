@@ -223,10 +223,52 @@ void ora_sssp_all(const int64_t *Va, const int64_t *Ea, const double *Wa,
 
 /* ---- link functions (tap_functions.c) ----------------------------------- */
 
+/* Power function used by BPR and the objective.  Mode 0 (default) is libm
+ * pow, as in the reference.  Mode 1 returns pow(x, 4) correctly rounded,
+ * computed independently of the product's code: Veltkamp/Dekker exact
+ * products (no fma; this file is built with -ffp-contract=off) give
+ * x^2 = h1 + l1 and h1^2 = h2 + l2 exactly, and x^4 = h2 + l2 + 2 h1 l1 + l1^2
+ * is rounded once.  glibc's pow(x, 4) is within 0.52 ulp but not always
+ * correctly rounded, so the two modes differ by one ulp on a small fraction
+ * of arguments; mode 1 lets tests compare the GPU engine bit for bit. */
+static int g_pow_mode = 0;
+
+void ora_set_pow_mode(int mode) { g_pow_mode = mode; }
+
+static void exact_mul(double a, double b, double *hi, double *lo)
+{
+  const double split = 134217729.0;            /* 2^27 + 1 */
+  const double p = a * b;
+  const double ca = split * a, ah = ca - (ca - a), al = a - ah;
+  const double cb = split * b, bh = cb - (cb - b), bl = b - bh;
+  *hi = p;
+  *lo = ((ah * bh - p) + ah * bl + al * bh) + al * bl;
+}
+
+static double pow4_rounded_once(double x)
+{
+  double h1, l1, h2, l2;
+  if (!(fabs(x) < 1e75) || fabs(x) < 1e-70)     /* outside the exact-split range */
+    return pow(x, 4.0);
+  exact_mul(x, x, &h1, &l1);
+  exact_mul(h1, h1, &h2, &l2);
+  const double tail = l2 + (2.0 * h1) * l1 + l1 * l1;
+  return h2 + tail;
+}
+
+double ora_pow4_exact_check(double x) { return pow4_rounded_once(x); }
+
+static double ora_pow(double x, double p)
+{
+  if (g_pow_mode == 1 && p == 4.0)
+    return pow4_rounded_once(x);
+  return pow(x, p);
+}
+
 /* BPR cost t0*(1 + B*(Q/c)^power), tap_functions.c:55-65 */
 double ora_bpr(double t0, double cap, double B, double power, double Q)
 {
-  return t0 * (1.0 + B * pow(Q / cap, power));
+  return t0 * (1.0 + B * ora_pow(Q / cap, power));
 }
 
 /* integral of BPR from 0 to Q in the reference's algebraic form,
@@ -235,7 +277,7 @@ double ora_integral(double t0, double cap, double B, double power, double Q)
 {
   if (!(power >= 0))
     return 0.0;
-  return Q * t0 * (1 + ((B / pow(cap, power)) / (power + 1)) * pow(Q, power));
+  return Q * t0 * (1 + ((B / ora_pow(cap, power)) / (power + 1)) * ora_pow(Q, power));
 }
 
 /* sum_k t_k(x_k + lambda d_k) d_k in link order, tap_functions.c:83-97 */

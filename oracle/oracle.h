@@ -66,6 +66,11 @@ void ora_sssp_all(const int64_t *Va, const int64_t *Ea, const double *Wa,
                   int64_t num_nodes, int64_t num_zones, int64_t first_thru_node,
                   int lll, double *dist, int64_t *pred, int nthreads);
 
+/* 0: libm pow (the reference's arithmetic, default); 1: pow(x,4) correctly
+   rounded (to compare the GPU engine bit for bit) */
+void ora_set_pow_mode(int mode);
+double ora_pow4_exact_check(double x);
+
 double ora_bpr(double t0, double cap, double B, double power, double Q);
 double ora_integral(double t0, double cap, double B, double power, double Q);
 double ora_dirderiv(const ora_net *net, const double *cap, const double *x,

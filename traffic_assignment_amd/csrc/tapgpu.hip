@@ -1,0 +1,1234 @@
+// tapgpu.hip -- gfx950 Frank-Wolfe user-equilibrium engine (C ABI in
+// include/tapgpu.h).
+//
+// Replaces the reference's per-iteration work (trunk/tap/tap_frankwolfe_*.c,
+// trunk/tap/pape_*.cu, trunk/tap/volume_update_*.cu) with HBM-resident state
+// and four kernel families, all FP64 and none of them MFMA-shaped:
+//
+//  * all-origin shortest paths: node-major labels dist[v][col] with one
+//    origin per wavefront lane (64 origins per "group"); pull relaxations
+//    over an explicit worklist of (node, group) items; a node's out-neighbours
+//    are queued for the next sweep when any lane improved, deduplicated by a
+//    per-(node, group) stamp and compacted with a wavefront ballot/prefix.
+//    Labels are the least fixed point of fl(dist[u] + w) over paths whose
+//    inner nodes are not zones (trunk/tap/sssp_pape.c:355), so they are
+//    bit-identical to d'Esopo-Pape's for any relaxation order.
+//  * all-or-nothing loading: bottom-up accumulation on the predecessor trees
+//    (children counted in a dense pass, then ready nodes push their subtree
+//    demand to the parent level by level); per-group partial link volumes are
+//    summed in a fixed order, so volumes are deterministic (and exact for
+//    integer demand, where the reference's path walk is exact too).
+//  * link passes: one fused pass evaluates g(0) = sum c*d, the Beckmann
+//    objective, VHT and g(2^-j) for a window of j (the only step sizes the
+//    reference's bisection ever tests, trunk/tap/tap_functions.c:469-507)
+//    with compensated block sums; a decision kernel per variant picks lambda
+//    and updates gap / relative gap; an update pass does x += lambda*d and
+//    BPR.
+//  * everything is per "variant": a batch of TAPs over one union topology
+//    (uneven link sets via a presence mask) runs in the same launches.
+//
+// Build: hipcc --offload-arch=gfx950 -O3 -ffp-contract=off (no FMA
+// contraction: every + and * is one IEEE rounding, as on the reference host).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tapgpu.h"
+#include "tapmath.h"
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;                 // 4 wavefronts
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kJ = 16;                      // step sizes 2^-j evaluated per link pass
+constexpr int kQ = 3 + kJ;                  // g0, objective, vht, g(2^-j)...
+constexpr int kMaxHalvings = 1001;          // lambda = 2^-1 ... 2^-1001
+constexpr int kLinkBlocks = 120;            // partial-sum blocks per variant
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+__device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// ---------------------------------------------------------------------------
+// kernel arguments
+// ---------------------------------------------------------------------------
+
+struct Graph {
+  int N, E, ft0;                  // ft0 = first_thru_node - 1 (0-based)
+  const int* fw_off;              // [N+1]
+  const int* fw_dst;              // [E]  head of link k (CSR edge k == link k)
+  const int* rv_off;              // [N+1] in-links of v
+  const int* rv_src;              // [E]  tail, reverse order
+  const int* rv_eid;              // [E]  link index, reverse order
+  const int* tail;                // [E]  tail of link k
+};
+
+struct Labels {
+  double* dist;                   // [N * P]   (flow during loading)
+  int* pred;                      // [N * P]
+  int* pend;                      // [N * P]   children left / -1 done
+  long long P;                    // row stride (columns allocated)
+  int Galloc;                     // group stride of stamp / items
+  const int* col_src;             // [P] origin node of each column, -1 = padding
+  const int* grp_var;             // [Galloc] variant of each group
+  const unsigned char* var_on;    // [V] or null (all on)
+  const double* rvw;              // [V * E] link costs in reverse-CSR order
+};
+
+struct Queues {
+  const unsigned* qin;
+  unsigned* qout;
+  int* cnt;                       // per-sweep item counters
+  int* stamp;                     // [N * Galloc]
+};
+
+__device__ __forceinline__ bool group_on(const Labels& L, int g) {
+  return L.var_on == nullptr || L.var_on[L.grp_var[g]] != 0;
+}
+
+// Queue the out-neighbours of v for group g (wave-cooperative).  Each (node,
+// group) enters a sweep's queue once: the first lane to swap the sweep's tag
+// into stamp[] appends it; appends are compacted with a ballot prefix.
+__device__ void enqueue_out(const Graph& G, const Queues& Q, int v, int g, int Galloc,
+                            int tag, int* cnt_next) {
+  const int lane = lane_id();
+  const int beg = G.fw_off[v], end = G.fw_off[v + 1];
+  for (int base = beg; base < end; base += kWave) {
+    const int e = base + lane;
+    bool first = false;
+    int x = 0;
+    if (e < end) {
+      x = G.fw_dst[e];
+      const int old = atomicExch(&Q.stamp[(size_t)x * Galloc + g], tag);
+      first = (old != tag);
+    }
+    const unsigned long long m = ballot(first);
+    if (m) {
+      int pos = 0;
+      if (lane == 0) pos = atomicAdd(cnt_next, (int)__popcll(m));
+      pos = __shfl(pos, 0);
+      if (first) {
+        const int rank = (int)__popcll(m & ((1ull << lane) - 1ull));
+        Q.qout[pos + rank] = (unsigned)x * (unsigned)Galloc + (unsigned)g;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// shortest paths
+// ---------------------------------------------------------------------------
+
+// dist = DBL_MAX, pred = -1 over the rows of groups [g0, g0+G).
+__global__ __launch_bounds__(kBlock) void k_fill_labels(Graph G, Labels L, int g0, int Gn) {
+  const long long waves = (long long)gridDim.x * kWavesPerBlock;
+  const long long items = (long long)G.N * Gn;
+  const int lane = lane_id();
+  for (long long it = (long long)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; it < items;
+       it += waves) {
+    const int v = (int)(it / Gn);
+    const int g = g0 + (int)(it - (long long)v * Gn);
+    if (!group_on(L, g)) continue;
+    const size_t idx = (size_t)v * L.P + (size_t)g * kWave + lane;
+    L.dist[idx] = DBL_MAX;
+    L.pred[idx] = -1;
+  }
+}
+
+// Origins get label 0; their out-neighbours form the first sweep's queue.
+__global__ __launch_bounds__(kBlock) void k_sssp_seed(Graph G, Labels L, Queues Q, int g0,
+                                                       int Gn, int tag) {
+  const int wave = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  if (wave >= Gn) return;
+  const int g = g0 + wave;
+  if (!group_on(L, g)) return;
+  const int col = g * kWave + lane_id();
+  const int src = L.col_src[col];
+  if (src < 0) return;
+  L.dist[(size_t)src * L.P + col] = 0.0;
+  for (int e = G.fw_off[src]; e < G.fw_off[src + 1]; ++e) {
+    const int x = G.fw_dst[e];
+    const int old = atomicExch(&Q.stamp[(size_t)x * L.Galloc + g], tag);
+    if (old != tag) {
+      const int pos = atomicAdd(&Q.cnt[0], 1);
+      Q.qout[pos] = (unsigned)x * (unsigned)L.Galloc + (unsigned)g;
+    }
+  }
+}
+
+// One sweep: every queued (v, g) pulls min over in-links of dist[u] + w for
+// its 64 origins (zones other than the lane's origin are not expanded).
+__global__ __launch_bounds__(kBlock) void k_sssp_sweep(Graph G, Labels L, Queues Q, int r,
+                                                        int tag) {
+  const int n = Q.cnt[r];
+  const int lane = lane_id();
+  const int waves = gridDim.x * kWavesPerBlock;
+  for (int it = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; it < n; it += waves) {
+    const unsigned item = (unsigned)uniform((int)Q.qin[it]);
+    const int v = uniform((int)(item / (unsigned)L.Galloc));
+    const int g = uniform((int)(item - (unsigned)v * (unsigned)L.Galloc));
+    const int col = g * kWave + lane;
+    const int src = L.col_src[col];
+    const double* __restrict__ w = L.rvw + (size_t)L.grp_var[g] * G.E;
+    const size_t me = (size_t)v * L.P + col;
+    double best = L.dist[me];
+    int arg = -1;
+    const int kb = G.rv_off[v], ke = G.rv_off[v + 1];
+    for (int k = kb; k < ke; ++k) {
+      const int u = G.rv_src[k];
+      if (u < G.ft0 && u != src) continue;          // zone: not a through node
+      const double du = L.dist[(size_t)u * L.P + col];
+      if (du < DBL_MAX) {
+        const double cand = du + w[k];
+        if (cand < best) {
+          best = cand;
+          arg = G.rv_eid[k];
+        }
+      }
+    }
+    const bool improved = arg >= 0;
+    if (improved) {
+      L.dist[me] = best;
+      L.pred[me] = arg;
+    }
+    if (ballot(improved && (v >= G.ft0 || v == src)))
+      enqueue_out(G, Q, v, g, L.Galloc, tag, &Q.cnt[r + 1]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// all-or-nothing loading on the predecessor trees
+// ---------------------------------------------------------------------------
+
+struct Loading {
+  const double* odc;     // [Z * Pv] demand origin(col) -> zone row
+  long long Pv;          // columns per variant
+  int Z;
+  double* ypart;         // [Galloc * E] per-group link volumes
+  int* err;              // [3] flag, origin, dest (0-based)
+};
+
+// Dense pass: children count, initial node flow (= demand to this zone),
+// no-path check, and the first ready set (leaves) as level 0.
+__global__ __launch_bounds__(kBlock) void k_aon_init(Graph G, Labels L, Loading A, Queues Q,
+                                                      int g0, int Gn) {
+  const long long waves = (long long)gridDim.x * kWavesPerBlock;
+  const long long items = (long long)G.N * Gn;
+  const int lane = lane_id();
+  for (long long it = (long long)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; it < items;
+       it += waves) {
+    const int v = uniform((int)(it / Gn));
+    const int g = uniform(g0 + (int)(it - (long long)v * Gn));
+    if (!group_on(L, g)) continue;
+    const int col = g * kWave + lane;
+    const int src = L.col_src[col];
+    const size_t me = (size_t)v * L.P + col;
+    const int p = L.pred[me];
+    int kids = 0;
+    for (int e = G.fw_off[v]; e < G.fw_off[v + 1]; ++e) {
+      const int x = G.fw_dst[e];
+      kids += (L.pred[(size_t)x * L.P + col] == e) ? 1 : 0;
+    }
+    double f = 0.0;
+    if (v < A.Z && src >= 0 && v != src) {
+      const double d = A.odc[(size_t)v * A.Pv + (col % A.Pv)];
+      if (d > 0.0) {
+        f = d;
+        if (p < 0 && atomicCAS(&A.err[0], 0, 1) == 0) {
+          A.err[1] = src;
+          A.err[2] = v;
+        }
+      }
+    }
+    L.dist[me] = f;
+    // pend >= 0: children still to report; -1: done / not in the tree;
+    // -(r+2): ready to be processed in level r (leaves: level 0)
+    L.pend[me] = (p < 0) ? -1 : (kids > 0 ? kids : -2);
+    if (ballot(p >= 0 && kids == 0) && lane == 0) {
+      const int pos = atomicAdd(&Q.cnt[0], 1);
+      Q.qout[pos] = (unsigned)v * (unsigned)L.Galloc + (unsigned)g;
+    }
+  }
+}
+
+// One level: ready lanes of each queued (v, g) add their subtree demand to
+// the in-link on their path, pass it to the tail node and release it.
+__global__ __launch_bounds__(kBlock) void k_aon_level(Graph G, Labels L, Loading A, Queues Q,
+                                                       int r, int tag) {
+  const int n = Q.cnt[r];
+  const int lane = lane_id();
+  const int waves = gridDim.x * kWavesPerBlock;
+  for (int it = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; it < n; it += waves) {
+    const unsigned item = (unsigned)uniform((int)Q.qin[it]);
+    const int v = uniform((int)(item / (unsigned)L.Galloc));
+    const int g = uniform((int)(item - (unsigned)v * (unsigned)L.Galloc));
+    const int col = g * kWave + lane;
+    const int src = L.col_src[col];
+    const size_t me = (size_t)v * L.P + col;
+    const int pe = L.pend[me];
+    const int p = L.pred[me];
+    // only lanes released for this level: their children's flow additions
+    // were made in earlier launches and are visible
+    const bool ready = (pe == -(r + 2)) && (p >= 0);
+    const double f = ready ? L.dist[me] : 0.0;
+    bool release = false;   // this lane's parent became ready
+    if (ready) {
+      const int parent = G.tail[p];
+      L.pend[me] = -1;
+      if (parent != src) {
+        const size_t pi = (size_t)parent * L.P + col;
+        if (f != 0.0) unsafeAtomicAdd(&L.dist[pi], f);
+        if (atomicSub(&L.pend[pi], 1) == 1) {
+          L.pend[pi] = -(r + 3);
+          release = true;
+        }
+      }
+    }
+    // per in-link: wave sum of the flow routed through it (fixed order)
+    double* __restrict__ yp = A.ypart + (size_t)g * G.E;
+    for (int k = G.rv_off[v]; k < G.rv_off[v + 1]; ++k) {
+      const int eid = G.rv_eid[k];
+      const bool mine = ready && p == eid;
+      if (!ballot(mine)) continue;
+      double s = mine ? f : 0.0;
+      for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+      const bool rel = ballot(mine && release) != 0ull;
+      if (lane == 0) {
+        if (s != 0.0) yp[eid] += s;
+      }
+      if (rel) {
+        const int u = G.rv_src[k];
+        if (lane == 0) {
+          const int old = atomicExch(&Q.stamp[(size_t)u * L.Galloc + g], tag);
+          if (old != tag) {
+            const int pos = atomicAdd(&Q.cnt[r + 1], 1);
+            Q.qout[pos] = (unsigned)u * (unsigned)L.Galloc + (unsigned)g;
+          }
+        }
+      }
+    }
+  }
+}
+
+// y[var][e] = sum over the variant's groups of ypart (fixed order); clears
+// ypart for the next loading.
+__global__ __launch_bounds__(kBlock) void k_aon_reduce(int E, int Gv, int v0, int Vn,
+                                                        const unsigned char* var_on,
+                                                        double* ypart, double* y) {
+  const long long n = (long long)E * Vn;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock) {
+    const int var = v0 + (int)(i / E);
+    const int e = (int)(i - (long long)(var - v0) * E);
+    if (var_on && !var_on[var]) continue;
+    double s = 0.0;
+    for (int j = 0; j < Gv; ++j) {
+      double* p = ypart + ((size_t)var * Gv + j) * E + e;
+      s += *p;
+      *p = 0.0;
+    }
+    y[(size_t)var * E + e] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// link passes
+// ---------------------------------------------------------------------------
+
+struct Links {
+  int E, V;
+  const double *t0, *cap, *B, *pw;   // [V * E]
+  const unsigned char* present;      // [V * E]
+  const int* rvpos;                  // [E] position of link k in reverse order
+  double *x, *y, *c;                 // [V * E]
+  double* rvw;                       // [V * E]
+};
+
+struct VarState {
+  double best_lb, relgap, lambda, objective, total_demand, vht;
+  int iter, active, stepping, need_more, jnext, found;
+};
+
+// mode 0: c = t0 (free-flow, initial Wa); mode 1: c = BPR(x); mode 2: c given
+__global__ __launch_bounds__(kBlock) void k_costs(Links K, int mode, const unsigned char* var_on,
+                                                  int v0, int Vn) {
+  const long long n = (long long)K.E * Vn;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock) {
+    const int var = v0 + (int)(i / K.E);
+    const int k = (int)(i - (long long)(var - v0) * K.E);
+    if (var_on && !var_on[var]) continue;
+    const size_t j = (size_t)var * K.E + k;
+    double c;
+    if (mode == 0) c = K.t0[j];
+    else if (mode == 1) c = tap_bpr(K.t0[j], K.cap[j], K.B[j], K.pw[j], K.x[j]);
+    else c = K.c[j];
+    K.c[j] = c;
+    K.rvw[(size_t)var * K.E + K.rvpos[k]] = K.present[j] ? c : INFINITY;
+  }
+}
+
+__device__ void block_reduce_acc(tap_acc* v, int nq, tap_acc* sh /* [kBlock] */) {
+  // fixed-order tree over the block, one quantity at a time
+  for (int q = 0; q < nq; ++q) {
+    sh[threadIdx.x] = v[q];
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) {
+        tap_acc a = sh[threadIdx.x];
+        tap_acc_merge(&a, sh[threadIdx.x + s]);
+        sh[threadIdx.x] = a;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) v[q] = sh[0];
+    __syncthreads();
+  }
+}
+
+// Partial sums per (variant, block): g0 = sum c*d, objective, vht and
+// g(2^-j) for j in [jbase, jbase + kJ) (j = 0 is lambda = 1).
+__global__ __launch_bounds__(kBlock) void k_link_partials(Links K, const VarState* st,
+                                                          tap_acc* part, int only_g) {
+  const int var = blockIdx.y;
+  const VarState s = st[var];
+  if (!s.stepping || (only_g && !s.need_more)) return;
+  const int jbase = only_g ? s.jnext : 0;
+  tap_acc acc[kQ];
+  for (int q = 0; q < kQ; ++q) acc[q] = tap_acc{0.0, 0.0};
+  double lam[kJ];
+  for (int j = 0; j < kJ; ++j) lam[j] = ldexp(1.0, -(jbase + j));
+  const size_t off = (size_t)var * K.E;
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < K.E; k += gridDim.x * kBlock) {
+    const size_t i = off + k;
+    if (!K.present[i]) continue;
+    const double x = K.x[i];
+    const double d = K.y[i] - x;
+    const double t0 = K.t0[i], cap = K.cap[i], B = K.B[i], pw = K.pw[i];
+    if (!only_g) {
+      const double c = K.c[i];
+      tap_acc_add(&acc[0], c * d);
+      tap_acc_add(&acc[1], tap_integral(t0, cap, B, pw, x));
+      tap_acc_add(&acc[2], x * c);
+    }
+    for (int j = 0; j < kJ; ++j) {
+      const double q = x + lam[j] * d;
+      tap_acc_add(&acc[3 + j], tap_bpr(t0, cap, B, pw, q) * d);
+    }
+  }
+  __shared__ tap_acc sh[kBlock];
+  block_reduce_acc(acc, kQ, sh);
+  if (threadIdx.x == 0) {
+    tap_acc* out = part + ((size_t)var * gridDim.x + blockIdx.x) * kQ;
+    for (int q = 0; q < kQ; ++q) out[q] = acc[q];
+  }
+}
+
+// Per variant: combine partials, pick lambda (first 2^-j with g <= 0, the
+// reference's rule), then gap, objective, lower bound, relative gap, average
+// excess cost, trace record, stop test.
+__global__ __launch_bounds__(kBlock) void k_link_decide(VarState* st, const tap_acc* part,
+                                                        int nparts, int only_g,
+                                                        tapgpu_iter* trace, int trace_cap,
+                                                        int max_iters, double target) {
+  const int var = blockIdx.x;
+  VarState s = st[var];
+  if (!s.stepping || (only_g && !s.need_more)) return;
+  tap_acc acc[kQ];
+  for (int q = 0; q < kQ; ++q) acc[q] = tap_acc{0.0, 0.0};
+  for (int b = threadIdx.x; b < nparts; b += kBlock) {
+    const tap_acc* in = part + ((size_t)var * nparts + b) * kQ;
+    for (int q = 0; q < kQ; ++q) tap_acc_merge(&acc[q], in[q]);
+  }
+  __shared__ tap_acc sh[kBlock];
+  block_reduce_acc(acc, kQ, sh);
+  if (threadIdx.x != 0) return;
+  double gj[kJ];
+  for (int j = 0; j < kJ; ++j) gj[j] = tap_acc_value(acc[3 + j]);
+  const int jbase = only_g ? s.jnext : 0;
+  if (!only_g) {
+    const double g0 = tap_acc_value(acc[0]);
+    s.objective = tap_acc_value(acc[1]);
+    s.vht = tap_acc_value(acc[2]);
+    s.found = 0;
+    s.need_more = 0;
+    if (g0 >= 0) {
+      s.lambda = 0.0; s.found = 1;
+    } else if (gj[0] <= 0) {
+      s.lambda = 1.0; s.found = 1;
+    }
+    // record the iteration's scalars (lambda filled below)
+    const double gap = -g0;
+    const double lb = s.objective - gap;
+    if (lb > s.best_lb) s.best_lb = lb;
+    if (s.best_lb != 0) s.relgap = (s.objective - s.best_lb) / fabs(s.best_lb);
+    if (s.iter < trace_cap) {
+      tapgpu_iter t;
+      t.lambda = 0.0; t.gap = gap; t.objective = s.objective; t.lowerbound = lb;
+      t.best_lowerbound = s.best_lb; t.relgap = s.relgap;
+      t.avg_excess = -g0 / s.total_demand; t.vht = s.vht;
+      trace[(size_t)var * trace_cap + s.iter] = t;
+    }
+  }
+  if (!s.found) {
+    for (int j = std::max(jbase, 1); j < jbase + kJ && j <= kMaxHalvings; ++j) {
+      if (gj[j - jbase] <= 0) {
+        s.lambda = ldexp(1.0, -j);
+        s.found = 1;
+        break;
+      }
+    }
+    if (!s.found) {
+      if (jbase + kJ > kMaxHalvings) {
+        s.lambda = 0.0;            // bisection cap reached: lambda_left == 0
+        s.found = 1;
+      } else {
+        s.need_more = 1;
+        s.jnext = jbase + kJ;
+      }
+    }
+  }
+  if (s.found) {
+    s.need_more = 0;
+    if (s.iter < trace_cap) trace[(size_t)var * trace_cap + s.iter].lambda = s.lambda;
+    s.iter += 1;
+    s.active = (max_iters <= 0 || s.iter < max_iters) && (target <= 0 || s.relgap > target);
+  }
+  st[var] = s;
+}
+
+// x += lambda * (y - x); c = BPR(x); reverse-order costs for the next sweep.
+__global__ __launch_bounds__(kBlock) void k_link_step(Links K, VarState* st) {
+  const int var = blockIdx.y;
+  if (!st[var].stepping) return;
+  const double lambda = st[var].lambda;
+  const size_t off = (size_t)var * K.E;
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < K.E; k += gridDim.x * kBlock) {
+    const size_t i = off + k;
+    const double d = K.y[i] - K.x[i];
+    const double x = K.x[i] + lambda * d;
+    K.x[i] = x;
+    const double c = tap_bpr(K.t0[i], K.cap[i], K.B[i], K.pw[i], x);
+    K.c[i] = c;
+    K.rvw[off + K.rvpos[k]] = K.present[i] ? c : INFINITY;
+  }
+}
+
+__global__ void k_finish_step(VarState* st, int V, unsigned char* var_on) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  st[v].stepping = 0;
+  var_on[v] = (unsigned char)st[v].active;
+}
+
+__global__ void k_begin_step(VarState* st, int V, const unsigned char* var_on) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  st[v].stepping = var_on[v] ? 1 : 0;
+  st[v].need_more = 0;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+};
+
+}  // namespace
+
+struct tapgpu_ctx {
+  std::string err;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  // sizes
+  int N = 0, E = 0, Z = 0, ft0 = 0, V = 1, Gv = 0, Galloc = 0;
+  long long Pv = 0, P = 0;
+  double total_demand = 0;
+  // graph
+  int *fw_off = nullptr, *fw_dst = nullptr, *rv_off = nullptr, *rv_src = nullptr,
+      *rv_eid = nullptr, *tail = nullptr, *rvpos = nullptr;
+  // link records per variant
+  double *t0 = nullptr, *cap = nullptr, *B = nullptr, *pw = nullptr;
+  unsigned char* present = nullptr;
+  double *x = nullptr, *y = nullptr, *c = nullptr, *rvw = nullptr;
+  // labels
+  double* dist = nullptr;
+  int *pred = nullptr, *pend = nullptr, *col_src = nullptr, *grp_var = nullptr;
+  int *stamp = nullptr, *cnt = nullptr;
+  unsigned *qa = nullptr, *qb = nullptr;
+  int cnt_cap = 0;
+  int tag = 0;
+  // loading
+  double *odc = nullptr, *ypart = nullptr;
+  int* err_dev = nullptr;
+  // FW state
+  unsigned char* var_on = nullptr;
+  VarState* st = nullptr;
+  tap_acc* part = nullptr;
+  tapgpu_iter* trace = nullptr;
+  int trace_cap = 0;
+  bool initialised = false;
+  // pinned host scratch
+  int* h_int = nullptr;
+  // stats
+  tapgpu_stats stats{};
+  std::vector<void*> allocs;
+
+  int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    err = buf;
+    return code;
+  }
+};
+
+namespace {
+
+std::string g_create_error;
+
+#define HIPCHK(ctx, call)                                                               \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return (ctx)->fail(TAPGPU_E_HIP, "%s failed: %s", #call, hipGetErrorString(e_));  \
+  } while (0)
+
+template <class T>
+int dalloc(tapgpu_ctx* ctx, T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return ctx->fail(TAPGPU_E_NOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T),
+                     hipGetErrorString(e));
+  }
+  ctx->allocs.push_back(*p);
+  return TAPGPU_OK;
+}
+
+template <class T>
+int upload(tapgpu_ctx* ctx, T* d, const T* h, size_t n) {
+  if (n == 0) return TAPGPU_OK;
+  HIPCHK(ctx, hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  return TAPGPU_OK;
+}
+
+int grid_for(long long waves) {
+  long long b = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+  if (b < 1) b = 1;
+  if (b > 8192) b = 8192;
+  return (int)b;
+}
+
+constexpr int kSweepBlocks = 2048;
+constexpr int kCheckEvery = 8;
+
+Graph make_graph(tapgpu_ctx* c) {
+  return Graph{c->N, c->E, c->ft0, c->fw_off, c->fw_dst, c->rv_off, c->rv_src, c->rv_eid, c->tail};
+}
+
+Labels make_labels(tapgpu_ctx* c, const int* col_src, const int* grp_var,
+                   const unsigned char* var_on) {
+  return Labels{c->dist, c->pred, c->pend, c->P, c->Galloc, col_src, grp_var, var_on, c->rvw};
+}
+
+Links make_links(tapgpu_ctx* c) {
+  return Links{c->E, c->V, c->t0, c->cap, c->B, c->pw, c->present, c->rvpos,
+               c->x, c->y, c->c, c->rvw};
+}
+
+// Worklist loop shared by the sweeps and the loading levels: launch
+// kCheckEvery rounds, then read the next round's queue length.
+template <class Launch>
+int run_rounds(tapgpu_ctx* c, Launch launch, int64_t* rounds_out, int64_t* items_out) {
+  int r = 0;
+  for (;;) {
+    if (r + kCheckEvery + 2 >= c->cnt_cap)
+      return c->fail(TAPGPU_E_LIMIT, "worklist did not drain in %d rounds", r);
+    for (int k = 0; k < kCheckEvery; ++k, ++r) {
+      unsigned* qin = (r & 1) ? c->qb : c->qa;
+      unsigned* qout = (r & 1) ? c->qa : c->qb;
+      launch(r, qin, qout, c->tag + r + 1);
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->h_int, c->cnt + r, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->h_int[0] == 0) break;
+  }
+  if (items_out) {
+    std::vector<int> h(r + 1);
+    HIPCHK(c, hipMemcpy(h.data(), c->cnt, sizeof(int) * (r + 1), hipMemcpyDeviceToHost));
+    int64_t s = 0;
+    int last = 0;
+    for (int i = 0; i <= r; ++i) {
+      s += h[i];
+      if (h[i]) last = i + 1;
+    }
+    *items_out += s;
+    if (rounds_out) *rounds_out += last;
+  }
+  c->tag += r + 2;
+  return TAPGPU_OK;
+}
+
+// tags are compared for equality with stamp[]; restart them long before
+// 32-bit overflow
+int maybe_reset_tags(tapgpu_ctx* c) {
+  if (c->tag < 2000000000) return TAPGPU_OK;
+  HIPCHK(c, hipMemsetAsync(c->stamp, 0xFF, sizeof(int) * (size_t)c->N * c->Galloc, c->stream));
+  c->tag = 0;
+  return TAPGPU_OK;
+}
+
+// Shortest paths for groups [g0, g0+Gn).
+int sssp_groups(tapgpu_ctx* c, int g0, int Gn, const int* col_src, const int* grp_var,
+                const unsigned char* var_on) {
+  int rc = maybe_reset_tags(c);
+  if (rc) return rc;
+  Graph G = make_graph(c);
+  Labels L = make_labels(c, col_src, grp_var, var_on);
+  HIPCHK(c, hipMemsetAsync(c->cnt, 0, sizeof(int) * c->cnt_cap, c->stream));
+  hipLaunchKernelGGL(k_fill_labels, dim3(grid_for((long long)c->N * Gn)), dim3(kBlock), 0,
+                     c->stream, G, L, g0, Gn);
+  Queues Q{c->qb, c->qa, c->cnt, c->stamp};
+  hipLaunchKernelGGL(k_sssp_seed, dim3((Gn + kWavesPerBlock - 1) / kWavesPerBlock), dim3(kBlock), 0,
+                     c->stream, G, L, Q, g0, Gn, c->tag);
+  auto t0 = std::chrono::steady_clock::now();
+  rc = run_rounds(
+      c,
+      [&](int r, unsigned* qin, unsigned* qout, int tag) {
+        Queues q{qin, qout, c->cnt, c->stamp};
+        hipLaunchKernelGGL(k_sssp_sweep, dim3(kSweepBlocks), dim3(kBlock), 0, c->stream, G, L, q,
+                           r, tag);
+      },
+      &c->stats.sssp_sweeps, &c->stats.sssp_items);
+  c->stats.sssp_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+// Loading for groups [g0, g0+Gn) of variants [v0, v0+Vn) into y.
+int aon_groups(tapgpu_ctx* c, int g0, int Gn, int v0, int Vn, const unsigned char* var_on) {
+  int rc = maybe_reset_tags(c);
+  if (rc) return rc;
+  auto t0 = std::chrono::steady_clock::now();
+  Graph G = make_graph(c);
+  Labels L = make_labels(c, c->col_src, c->grp_var, var_on);
+  Loading A{c->odc, c->Pv, c->Z, c->ypart, c->err_dev};
+  HIPCHK(c, hipMemsetAsync(c->cnt, 0, sizeof(int) * c->cnt_cap, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
+  Queues Q0{nullptr, c->qa, c->cnt, c->stamp};
+  hipLaunchKernelGGL(k_aon_init, dim3(grid_for((long long)c->N * Gn)), dim3(kBlock), 0, c->stream,
+                     G, L, A, Q0, g0, Gn);
+  // level r reads qa when r is even (the init kernel wrote qa)
+  rc = run_rounds(
+      c,
+      [&](int r, unsigned* qb_in, unsigned* qa_out, int tag) {
+        // run_rounds hands (qa, qb) for even r as (in=qa, out=qb)
+        Queues q{qb_in, qa_out, c->cnt, c->stamp};
+        hipLaunchKernelGGL(k_aon_level, dim3(kSweepBlocks), dim3(kBlock), 0, c->stream, G, L, A, q,
+                           r, tag);
+      },
+      &c->stats.aon_levels, &c->stats.aon_items);
+  if (rc) return rc;
+  int herr[3];
+  HIPCHK(c, hipMemcpyAsync(herr, c->err_dev, sizeof herr, hipMemcpyDeviceToHost, c->stream));
+  hipLaunchKernelGGL(k_aon_reduce, dim3(grid_for((long long)c->E * Vn / kWave + 1)), dim3(kBlock), 0,
+                     c->stream, c->E, c->Gv, v0, Vn, var_on, c->ypart, c->y);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->stats.aon_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (herr[0])
+    return c->fail(TAPGPU_E_NOPATH, "no path from %d to %d", herr[1] + 1, herr[2] + 1);
+  return TAPGPU_OK;
+}
+
+int launch_costs(tapgpu_ctx* c, int mode, const unsigned char* var_on, int v0, int Vn) {
+  hipLaunchKernelGGL(k_costs, dim3(grid_for((long long)c->E * Vn / kWave + 1)), dim3(kBlock), 0,
+                     c->stream, make_links(c), mode, var_on, v0, Vn);
+  HIPCHK(c, hipGetLastError());
+  return TAPGPU_OK;
+}
+
+int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, int64_t V,
+          const uint8_t* present, const double* cap_v, const double* t0_v, const double* B_v,
+          const double* pw_v) {
+  if (!net || !dem) return c->fail(TAPGPU_E_ARG, "null network or demand");
+  const int64_t N = net->num_nodes, E = net->num_links, Z = net->num_zones;
+  if (N < 1 || E < 0 || Z < 0 || Z > N || V < 1)
+    return c->fail(TAPGPU_E_ARG, "bad sizes N=%lld E=%lld Z=%lld V=%lld", (long long)N,
+                   (long long)E, (long long)Z, (long long)V);
+  if (N >= (1LL << 30) || E >= (1LL << 30))
+    return c->fail(TAPGPU_E_LIMIT, "network too large for 32-bit indices");
+  if (dem->num_zones != Z)
+    return c->fail(TAPGPU_E_ARG, "demand has %lld zones, network %lld", (long long)dem->num_zones,
+                   (long long)Z);
+  c->N = (int)N; c->E = (int)E; c->Z = (int)Z; c->V = (int)V;
+  int64_t ft = net->first_thru_node;
+  if (ft < 1) ft = 1;
+  if (ft > N + 1) ft = N + 1;
+  c->ft0 = (int)(ft - 1);
+  c->Gv = (int)((std::max<int64_t>(Z, 1) + kWave - 1) / kWave);
+  c->Pv = (long long)c->Gv * kWave;
+  c->Galloc = c->Gv * c->V;
+  c->P = c->Pv * c->V;
+  if ((unsigned long long)N * (unsigned long long)c->Galloc >= (1ull << 32))
+    return c->fail(TAPGPU_E_LIMIT, "nodes x origin groups exceeds 2^32");
+
+  // --- graph (host build) ---
+  std::vector<int> fw_off(N + 1, 0), fw_dst(E), rv_off(N + 1, 0), rv_src(E), rv_eid(E), tail(E),
+      rvpos(E);
+  for (int64_t k = 0; k < E; ++k) {
+    const int64_t a = net->init_node[k], b = net->term_node[k];
+    if (a < 1 || a > N || b < 1 || b > N)
+      return c->fail(TAPGPU_E_ARG, "link %lld has node out of range (%lld, %lld)", (long long)k,
+                     (long long)a, (long long)b);
+    if (k > 0 && (a < net->init_node[k - 1] ||
+                  (a == net->init_node[k - 1] && b < net->term_node[k - 1])))
+      return c->fail(TAPGPU_E_ARG, "links not sorted by (init, term) at %lld", (long long)k);
+    fw_off[a]++;  // count into slot a (0-based a-1 -> prefix below)
+    rv_off[b]++;
+    fw_dst[k] = (int)(b - 1);
+    tail[k] = (int)(a - 1);
+  }
+  for (int64_t v = 0; v < N; ++v) {
+    fw_off[v + 1] += fw_off[v];
+    rv_off[v + 1] += rv_off[v];
+  }
+  {
+    std::vector<int> fill(rv_off.begin(), rv_off.end() - 1);
+    for (int64_t k = 0; k < E; ++k) {   // stable: in-links of v in link order
+      const int b = fw_dst[k];
+      const int pos = fill[b]++;
+      rv_src[pos] = tail[k];
+      rv_eid[pos] = (int)k;
+      rvpos[k] = pos;
+    }
+  }
+  // --- demand: dense per zone row in column order ---
+  std::vector<double> odc((size_t)std::max<int64_t>(Z, 1) * c->Pv, 0.0);
+  double tot = 0;
+  for (int64_t o = 1; o <= Z; ++o) {
+    for (int64_t i = dem->od_start[o - 1]; i < dem->od_start[o]; ++i) {
+      const int64_t d = dem->od_dest[i];
+      const double f = dem->od_demand[i];
+      tot += f;
+      if (d == o || !(f > 0)) continue;
+      if (d < 1 || d > Z)
+        return c->fail(TAPGPU_E_ARG, "demand destination %lld is not a zone", (long long)d);
+      odc[(size_t)(d - 1) * c->Pv + (o - 1)] += f;
+    }
+  }
+  c->total_demand = tot;
+
+  // --- link records per variant (capacity fix-up per variant) ---
+  const size_t VE = (size_t)V * E;
+  std::vector<double> hcap(VE), ht0(VE), hB(VE), hpw(VE);
+  std::vector<unsigned char> hpres(VE, 1);
+  for (int64_t v = 0; v < V; ++v) {
+    double minpos = DBL_MAX;
+    for (int64_t k = 0; k < E; ++k) {
+      const size_t j = (size_t)v * E + k;
+      hcap[j] = cap_v ? cap_v[j] : net->capacity[k];
+      ht0[j] = t0_v ? t0_v[j] : net->free_flow_time[k];
+      hB[j] = B_v ? B_v[j] : net->B[k];
+      hpw[j] = pw_v ? pw_v[j] : net->power[k];
+      if (present) hpres[j] = present[j] ? 1 : 0;
+      if (hpres[j] && hcap[j] > 0.0 && hcap[j] < minpos) minpos = hcap[j];
+    }
+    for (int64_t k = 0; k < E; ++k) {
+      const size_t j = (size_t)v * E + k;
+      if (!(hcap[j] > 0.0)) hcap[j] = minpos;
+    }
+  }
+  std::vector<int> col_src(c->P, -1), grp_var(c->Galloc);
+  for (int v = 0; v < c->V; ++v) {
+    for (int64_t z = 0; z < Z; ++z) col_src[(size_t)v * c->Pv + z] = (int)z;
+    for (int g = 0; g < c->Gv; ++g) grp_var[v * c->Gv + g] = v;
+  }
+
+  // --- device ---
+  HIPCHK(c, hipGetDevice(&c->device));
+  HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(c, hipHostMalloc((void**)&c->h_int, 64 * sizeof(int), hipHostMallocDefault));
+  int rc;
+#define ALLOC(ptr, n) if ((rc = dalloc(c, &c->ptr, (n)))) return rc
+  ALLOC(fw_off, N + 1); ALLOC(fw_dst, E); ALLOC(rv_off, N + 1); ALLOC(rv_src, E);
+  ALLOC(rv_eid, E); ALLOC(tail, E); ALLOC(rvpos, E);
+  ALLOC(t0, VE); ALLOC(cap, VE); ALLOC(B, VE); ALLOC(pw, VE); ALLOC(present, VE);
+  ALLOC(x, VE); ALLOC(y, VE); ALLOC(c, VE); ALLOC(rvw, VE);
+  const size_t NP = (size_t)N * c->P;
+  ALLOC(dist, NP); ALLOC(pred, NP); ALLOC(pend, NP);
+  ALLOC(col_src, c->P); ALLOC(grp_var, c->Galloc);
+  ALLOC(stamp, (size_t)N * c->Galloc);
+  ALLOC(qa, (size_t)N * c->Galloc); ALLOC(qb, (size_t)N * c->Galloc);
+  c->cnt_cap = (int)std::min<int64_t>(std::max<int64_t>(4 * N + 64, 1024), 1 << 24);
+  ALLOC(cnt, c->cnt_cap);
+  ALLOC(odc, (size_t)std::max<int64_t>(Z, 1) * c->Pv);
+  ALLOC(ypart, (size_t)c->Galloc * E);
+  ALLOC(err_dev, 4);
+  ALLOC(var_on, V); ALLOC(st, V);
+  ALLOC(part, (size_t)V * kLinkBlocks * kQ);
+#undef ALLOC
+  if ((rc = upload(c, c->fw_off, fw_off.data(), N + 1))) return rc;
+  if ((rc = upload(c, c->fw_dst, fw_dst.data(), E))) return rc;
+  if ((rc = upload(c, c->rv_off, rv_off.data(), N + 1))) return rc;
+  if ((rc = upload(c, c->rv_src, rv_src.data(), E))) return rc;
+  if ((rc = upload(c, c->rv_eid, rv_eid.data(), E))) return rc;
+  if ((rc = upload(c, c->tail, tail.data(), E))) return rc;
+  if ((rc = upload(c, c->rvpos, rvpos.data(), E))) return rc;
+  if ((rc = upload(c, c->t0, ht0.data(), VE))) return rc;
+  if ((rc = upload(c, c->cap, hcap.data(), VE))) return rc;
+  if ((rc = upload(c, c->B, hB.data(), VE))) return rc;
+  if ((rc = upload(c, c->pw, hpw.data(), VE))) return rc;
+  if ((rc = upload(c, c->present, hpres.data(), VE))) return rc;
+  if ((rc = upload(c, c->col_src, col_src.data(), c->P))) return rc;
+  if ((rc = upload(c, c->grp_var, grp_var.data(), c->Galloc))) return rc;
+  if ((rc = upload(c, c->odc, odc.data(), odc.size()))) return rc;
+  HIPCHK(c, hipMemsetAsync(c->stamp, 0xFF, sizeof(int) * (size_t)N * c->Galloc, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->ypart, 0, sizeof(double) * (size_t)c->Galloc * E, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->x, 0, sizeof(double) * VE, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->y, 0, sizeof(double) * VE, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->var_on, 1, V, c->stream));
+  // Wa = free-flow time (trunk/tap/tap_functions.c:331)
+  if ((rc = launch_costs(c, 0, nullptr, 0, c->V))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TAPGPU_OK;
+}
+
+int check_variant(tapgpu_ctx* c, int64_t v) {
+  if (v < 0 || v >= c->V) return c->fail(TAPGPU_E_ARG, "variant %lld out of range", (long long)v);
+  return TAPGPU_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int tapgpu_create_batched(tapgpu_ctx** out, const tapgpu_network* net, const tapgpu_demand* dem,
+                          int64_t num_variants, const uint8_t* present, const double* capacity_v,
+                          const double* free_flow_time_v, const double* B_v,
+                          const double* power_v) {
+  if (!out) return TAPGPU_E_ARG;
+  *out = nullptr;
+  tapgpu_ctx* c = new (std::nothrow) tapgpu_ctx();
+  if (!c) return TAPGPU_E_NOMEM;
+  const int rc = build(c, net, dem, num_variants, present, capacity_v, free_flow_time_v, B_v,
+                       power_v);
+  if (rc) {
+    g_create_error = c->err;
+    tapgpu_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return TAPGPU_OK;
+}
+
+int tapgpu_create(tapgpu_ctx** out, const tapgpu_network* net, const tapgpu_demand* dem) {
+  return tapgpu_create_batched(out, net, dem, 1, nullptr, nullptr, nullptr, nullptr, nullptr);
+}
+
+void tapgpu_destroy(tapgpu_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->trace) (void)hipFree(c->trace);
+  if (c->h_int) (void)hipHostFree(c->h_int);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* tapgpu_last_error(const tapgpu_ctx* c) {
+  return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+int64_t tapgpu_num_variants(const tapgpu_ctx* c) { return c ? c->V : 0; }
+
+int tapgpu_set_costs(tapgpu_ctx* c, int64_t variant, const double* costs) {
+  if (!c || !costs) return TAPGPU_E_ARG;
+  int rc = check_variant(c, variant);
+  if (rc) return rc;
+  if ((rc = upload(c, c->c + (size_t)variant * c->E, costs, c->E))) return rc;
+  if ((rc = launch_costs(c, 2, nullptr, (int)variant, 1))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TAPGPU_OK;
+}
+
+int tapgpu_sssp(tapgpu_ctx* c, int64_t variant, const int64_t* start_nodes, int64_t num_start,
+                double* dist, int64_t* pred, int64_t ld, int64_t col_base) {
+  if (!c || (num_start > 0 && !start_nodes) || num_start < 0 || col_base < 0)
+    return TAPGPU_E_ARG;
+  int rc = check_variant(c, variant);
+  if (rc) return rc;
+  if (ld < col_base + num_start)
+    return c->fail(TAPGPU_E_ARG, "ld %lld < col_base + num_start", (long long)ld);
+  for (int64_t i = 0; i < num_start; ++i)
+    if (start_nodes[i] < 1 || start_nodes[i] > c->N)
+      return c->fail(TAPGPU_E_ARG, "start node %lld out of range", (long long)start_nodes[i]);
+  const int64_t N1 = (int64_t)c->N + 1;
+  // node 0 does not exist: row 0 stays unreachable
+  for (int64_t i = 0; i < num_start; ++i) {
+    if (dist) dist[col_base + i] = DBL_MAX;
+    if (pred) pred[col_base + i] = -1;
+  }
+  // chunks of the variant's columns; groups [0, Gv) of a private mapping
+  int* d_col = nullptr;
+  int* d_gv = nullptr;
+  if ((rc = dalloc(c, &d_col, c->Pv))) return rc;
+  if ((rc = dalloc(c, &d_gv, c->Gv))) return rc;
+  std::vector<int> hgv(c->Gv, (int)variant), hcol(c->Pv);
+  std::vector<double> hd((size_t)c->N * c->Pv);
+  std::vector<int> hp((size_t)c->N * c->Pv);
+  rc = upload(c, d_gv, hgv.data(), c->Gv);
+  for (int64_t s0 = 0; !rc && s0 < num_start; s0 += c->Pv) {
+    const int64_t ns = std::min<int64_t>(c->Pv, num_start - s0);
+    for (int64_t j = 0; j < c->Pv; ++j) hcol[j] = j < ns ? (int)(start_nodes[s0 + j] - 1) : -1;
+    if ((rc = upload(c, d_col, hcol.data(), c->Pv))) break;
+    const int Gn = (int)((ns + kWave - 1) / kWave);
+    if ((rc = sssp_groups(c, 0, Gn, d_col, d_gv, nullptr))) break;
+    // rows are P apart on the device; copy the first Pv columns of each row
+    const hipError_t e1 = hipMemcpy2DAsync(hd.data(), c->Pv * sizeof(double), c->dist,
+                                           c->P * sizeof(double), c->Pv * sizeof(double), c->N,
+                                           hipMemcpyDeviceToHost, c->stream);
+    const hipError_t e2 = hipMemcpy2DAsync(hp.data(), c->Pv * sizeof(int), c->pred,
+                                           c->P * sizeof(int), c->Pv * sizeof(int), c->N,
+                                           hipMemcpyDeviceToHost, c->stream);
+    const hipError_t e3 = hipStreamSynchronize(c->stream);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+      rc = c->fail(TAPGPU_E_HIP, "copy of labels failed");
+      break;
+    }
+    for (int64_t v = 1; v < N1; ++v) {
+      const size_t row = (size_t)(v - 1) * c->Pv;
+      for (int64_t j = 0; j < ns; ++j) {
+        const size_t o = (size_t)v * ld + col_base + s0 + j;
+        if (dist) dist[o] = hd[row + j];
+        if (pred) pred[o] = hp[row + j];
+      }
+    }
+  }
+  // private buffers are freed here (not kept in allocs)
+  c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void*)d_col), c->allocs.end());
+  c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void*)d_gv), c->allocs.end());
+  (void)hipFree(d_col);
+  (void)hipFree(d_gv);
+  return rc;
+}
+
+int tapgpu_aon(tapgpu_ctx* c, int64_t variant, double* volumes) {
+  if (!c) return TAPGPU_E_ARG;
+  int rc = check_variant(c, variant);
+  if (rc) return rc;
+  const int v = (int)variant;
+  std::vector<unsigned char> on(c->V, 0);
+  on[v] = 1;
+  unsigned char* d_on = nullptr;
+  if ((rc = dalloc(c, &d_on, c->V))) return rc;
+  rc = upload(c, d_on, on.data(), c->V);
+  if (!rc) rc = sssp_groups(c, v * c->Gv, c->Gv, c->col_src, c->grp_var, d_on);
+  if (!rc) rc = aon_groups(c, v * c->Gv, c->Gv, v, 1, d_on);
+  if (!rc && volumes) {
+    const hipError_t e = hipMemcpy(volumes, c->y + (size_t)v * c->E, sizeof(double) * c->E,
+                                   hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = c->fail(TAPGPU_E_HIP, "copy of volumes failed");
+  }
+  c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void*)d_on), c->allocs.end());
+  (void)hipFree(d_on);
+  return rc;
+}
+
+static int ensure_trace(tapgpu_ctx* c, int need) {
+  if (need <= c->trace_cap) return TAPGPU_OK;
+  int cap = std::max(need, 1024);
+  tapgpu_iter* t = nullptr;
+  hipError_t e = hipMalloc((void**)&t, sizeof(tapgpu_iter) * (size_t)cap * c->V);
+  if (e != hipSuccess) return c->fail(TAPGPU_E_NOMEM, "trace allocation failed");
+  HIPCHK(c, hipMemsetAsync(t, 0, sizeof(tapgpu_iter) * (size_t)cap * c->V, c->stream));
+  if (c->trace) {
+    for (int v = 0; v < c->V; ++v)
+      HIPCHK(c, hipMemcpyAsync(t + (size_t)v * cap, c->trace + (size_t)v * c->trace_cap,
+                               sizeof(tapgpu_iter) * c->trace_cap, hipMemcpyDeviceToDevice,
+                               c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(c->trace);
+  }
+  c->trace = t;
+  c->trace_cap = cap;
+  return TAPGPU_OK;
+}
+
+int tapgpu_fw_init(tapgpu_ctx* c, const double* warm_volumes) {
+  if (!c) return TAPGPU_E_ARG;
+  int rc = ensure_trace(c, 1024);
+  if (rc) return rc;
+  std::vector<VarState> hs(c->V);
+  for (auto& s : hs) {
+    s = VarState{};
+    s.best_lb = -DBL_MAX;
+    s.relgap = DBL_MAX;
+    s.total_demand = c->total_demand;
+    s.active = 1;
+  }
+  if ((rc = upload(c, c->st, hs.data(), c->V))) return rc;
+  HIPCHK(c, hipMemsetAsync(c->var_on, 1, c->V, c->stream));
+  if (warm_volumes) {
+    if ((rc = upload(c, c->x, warm_volumes, (size_t)c->V * c->E))) return rc;
+  } else {
+    if ((rc = launch_costs(c, 0, nullptr, 0, c->V))) return rc;   // Wa = t0
+    if ((rc = sssp_groups(c, 0, c->Galloc, c->col_src, c->grp_var, nullptr))) return rc;
+    if ((rc = aon_groups(c, 0, c->Galloc, 0, c->V, nullptr))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->x, c->y, sizeof(double) * (size_t)c->V * c->E,
+                             hipMemcpyDeviceToDevice, c->stream));
+  }
+  if ((rc = launch_costs(c, 1, nullptr, 0, c->V))) return rc;      // c = BPR(x)
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->initialised = true;
+  return TAPGPU_OK;
+}
+
+int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_result* results) {
+  if (!c) return TAPGPU_E_ARG;
+  if (!c->initialised) return c->fail(TAPGPU_E_STATE, "tapgpu_fw_init was not called");
+  int rc = ensure_trace(c, max_iters > 0 ? max_iters : 1024);
+  if (rc) return rc;
+  c->stats = tapgpu_stats{};
+  auto t_start = std::chrono::steady_clock::now();
+  Links K = make_links(c);
+  const dim3 lgrid(kLinkBlocks, c->V);
+  std::vector<unsigned char> hon(c->V);
+  std::vector<VarState> hs(c->V);
+  // variants already past the stop rule do not run (the loop test comes first)
+  HIPCHK(c, hipMemcpy(hs.data(), c->st, sizeof(VarState) * c->V, hipMemcpyDeviceToHost));
+  for (int v = 0; v < c->V; ++v) {
+    hs[v].active = (max_iters <= 0 || hs[v].iter < max_iters) &&
+                   (target_relgap <= 0 || hs[v].relgap > target_relgap);
+    hon[v] = (unsigned char)hs[v].active;
+  }
+  if ((rc = upload(c, c->st, hs.data(), c->V))) return rc;
+  if ((rc = upload(c, c->var_on, hon.data(), c->V))) return rc;
+  for (;;) {
+    bool any = false;
+    for (int v = 0; v < c->V; ++v) any = any || hon[v];
+    if (!any) break;
+    if (c->trace_cap < 1 << 30) {
+      int maxit = 0;
+      for (int v = 0; v < c->V; ++v) maxit = std::max(maxit, hs[v].iter);
+      if (maxit + 1 >= c->trace_cap && (rc = ensure_trace(c, 2 * c->trace_cap))) return rc;
+    }
+    hipLaunchKernelGGL(k_begin_step, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->st, c->V,
+                       c->var_on);
+    if ((rc = sssp_groups(c, 0, c->Galloc, c->col_src, c->grp_var, c->var_on))) return rc;
+    if ((rc = aon_groups(c, 0, c->Galloc, 0, c->V, c->var_on))) return rc;
+    auto tl = std::chrono::steady_clock::now();
+    hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kBlock), 0, c->stream, K, c->st, c->part, 0);
+    hipLaunchKernelGGL(k_link_decide, dim3(c->V), dim3(kBlock), 0, c->stream, c->st, c->part,
+                       kLinkBlocks, 0, c->trace, c->trace_cap, max_iters, target_relgap);
+    c->stats.link_passes++;
+    for (int extra = 0;; ++extra) {
+      HIPCHK(c, hipMemcpyAsync(hs.data(), c->st, sizeof(VarState) * c->V, hipMemcpyDeviceToHost,
+                               c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      bool more = false;
+      for (int v = 0; v < c->V; ++v) more = more || (hs[v].stepping && hs[v].need_more);
+      if (!more) break;
+      hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kBlock), 0, c->stream, K, c->st, c->part, 1);
+      hipLaunchKernelGGL(k_link_decide, dim3(c->V), dim3(kBlock), 0, c->stream, c->st, c->part,
+                         kLinkBlocks, 1, c->trace, c->trace_cap, max_iters, target_relgap);
+      c->stats.link_passes++;
+    }
+    hipLaunchKernelGGL(k_link_step, lgrid, dim3(kBlock), 0, c->stream, K, c->st);
+    hipLaunchKernelGGL(k_finish_step, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->st,
+                       c->V, c->var_on);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(hon.data(), c->var_on, c->V, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->stats.link_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tl).count();
+  }
+  c->stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  if (results) {
+    HIPCHK(c, hipMemcpy(hs.data(), c->st, sizeof(VarState) * c->V, hipMemcpyDeviceToHost));
+    std::vector<double> x((size_t)c->E), cc((size_t)c->E);
+    for (int v = 0; v < c->V; ++v) {
+      tapgpu_result& r = results[v];
+      r.iterations = hs[v].iter;
+      r.relgap = hs[v].relgap;
+      r.objective = hs[v].objective;
+      r.converged = (target_relgap > 0 && hs[v].relgap <= target_relgap) ? 1 : 0;
+      // VHT after the last step: sum x*c over present links (host, fixed order)
+      HIPCHK(c, hipMemcpy(x.data(), c->x + (size_t)v * c->E, sizeof(double) * c->E,
+                          hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpy(cc.data(), c->c + (size_t)v * c->E, sizeof(double) * c->E,
+                          hipMemcpyDeviceToHost));
+      tap_acc a{0.0, 0.0};
+      for (int k = 0; k < c->E; ++k) tap_acc_add(&a, x[k] * cc[k]);
+      r.vht = tap_acc_value(a);
+    }
+  }
+  return TAPGPU_OK;
+}
+
+int tapgpu_get_flows(tapgpu_ctx* c, int64_t variant, double* volumes, double* costs) {
+  if (!c) return TAPGPU_E_ARG;
+  int rc = check_variant(c, variant);
+  if (rc) return rc;
+  if (volumes)
+    HIPCHK(c, hipMemcpy(volumes, c->x + (size_t)variant * c->E, sizeof(double) * c->E,
+                        hipMemcpyDeviceToHost));
+  if (costs)
+    HIPCHK(c, hipMemcpy(costs, c->c + (size_t)variant * c->E, sizeof(double) * c->E,
+                        hipMemcpyDeviceToHost));
+  return TAPGPU_OK;
+}
+
+int tapgpu_get_trace(tapgpu_ctx* c, int64_t variant, tapgpu_iter* trace, int cap, int* count) {
+  if (!c) return TAPGPU_E_ARG;
+  int rc = check_variant(c, variant);
+  if (rc) return rc;
+  VarState s;
+  HIPCHK(c, hipMemcpy(&s, c->st + variant, sizeof s, hipMemcpyDeviceToHost));
+  const int held = std::min(s.iter, c->trace_cap);
+  if (count) *count = s.iter;
+  const int n = std::min(held, cap);
+  if (trace && n > 0)
+    HIPCHK(c, hipMemcpy(trace, c->trace + (size_t)variant * c->trace_cap, sizeof(tapgpu_iter) * n,
+                        hipMemcpyDeviceToHost));
+  return TAPGPU_OK;
+}
+
+int tapgpu_device_state(tapgpu_ctx* c, const double** d_volumes, const double** d_costs) {
+  if (!c) return TAPGPU_E_ARG;
+  if (d_volumes) *d_volumes = c->x;
+  if (d_costs) *d_costs = c->c;
+  return TAPGPU_OK;
+}
+
+int tapgpu_get_stats(tapgpu_ctx* c, tapgpu_stats* s) {
+  if (!c || !s) return TAPGPU_E_ARG;
+  *s = c->stats;
+  return TAPGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,224 @@
+"""Python front end of the gfx950 engine (ctypes over include/tapgpu.h).
+
+Loads the in-tree libtapgpu.so built by traffic_assignment_amd.build; there
+is no fallback: if the library or a GPU is missing, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtapgpu.so")
+
+TAPGPU_OK = 0
+ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "no path", -4: "out of device memory",
+          -5: "bad call order", -6: "limit exceeded"}
+
+
+class TapGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class NoPathError(TapGpuError):
+    pass
+
+
+class _Network(C.Structure):
+    _fields_ = [("num_zones", C.c_int64), ("num_nodes", C.c_int64),
+                ("first_thru_node", C.c_int64), ("num_links", C.c_int64),
+                ("init_node", C.c_void_p), ("term_node", C.c_void_p),
+                ("capacity", C.c_void_p), ("free_flow_time", C.c_void_p),
+                ("B", C.c_void_p), ("power", C.c_void_p)]
+
+
+class _Demand(C.Structure):
+    _fields_ = [("num_zones", C.c_int64), ("od_start", C.c_void_p), ("od_dest", C.c_void_p),
+                ("od_demand", C.c_void_p)]
+
+
+ITER_FIELDS = ("lambda_", "gap", "objective", "lowerbound", "best_lowerbound", "relgap",
+               "avg_excess", "vht")
+
+
+class _Iter(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ITER_FIELDS]
+
+
+class _Result(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("relgap", C.c_double),
+                ("objective", C.c_double), ("vht", C.c_double)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("sssp_ms", C.c_double), ("aon_ms", C.c_double), ("link_ms", C.c_double),
+                ("total_ms", C.c_double), ("sssp_sweeps", C.c_int64), ("aon_levels", C.c_int64),
+                ("sssp_items", C.c_int64), ("aon_items", C.c_int64), ("link_passes", C.c_int64)]
+
+
+_lib = None
+vp = C.c_void_p
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built (run traffic_assignment_amd.build)")
+        L = C.CDLL(LIB_PATH)
+        L.tapgpu_create.argtypes = [C.POINTER(vp), C.POINTER(_Network), C.POINTER(_Demand)]
+        L.tapgpu_create_batched.argtypes = [C.POINTER(vp), C.POINTER(_Network), C.POINTER(_Demand),
+                                            C.c_int64, vp, vp, vp, vp, vp]
+        L.tapgpu_destroy.argtypes = [vp]
+        L.tapgpu_destroy.restype = None
+        L.tapgpu_last_error.argtypes = [vp]
+        L.tapgpu_last_error.restype = C.c_char_p
+        L.tapgpu_num_variants.argtypes = [vp]
+        L.tapgpu_num_variants.restype = C.c_int64
+        L.tapgpu_set_costs.argtypes = [vp, C.c_int64, vp]
+        L.tapgpu_sssp.argtypes = [vp, C.c_int64, vp, C.c_int64, vp, vp, C.c_int64, C.c_int64]
+        L.tapgpu_aon.argtypes = [vp, C.c_int64, vp]
+        L.tapgpu_fw_init.argtypes = [vp, vp]
+        L.tapgpu_fw_run.argtypes = [vp, C.c_int, C.c_double, vp]
+        L.tapgpu_get_flows.argtypes = [vp, C.c_int64, vp, vp]
+        L.tapgpu_get_trace.argtypes = [vp, C.c_int64, vp, C.c_int, C.POINTER(C.c_int)]
+        L.tapgpu_device_state.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
+        L.tapgpu_get_stats.argtypes = [vp, C.POINTER(_Stats)]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class TapEngine:
+    """One network topology, num_variants TAP instances, state in HBM.
+
+    net / dem follow traffic_assignment_amd.synth (1-based ids, links sorted
+    by (init, term)).  For the batched form pass per-variant arrays shaped
+    [num_variants, num_links] (present: bool mask over the union links).
+    """
+
+    def __init__(self, net, dem, num_variants=1, present=None, capacity=None,
+                 fftime=None, B=None, power=None):
+        L = lib()
+        self._keep = [np.ascontiguousarray(a, dtype=t) for a, t in (
+            (net.init, np.int64), (net.term, np.int64), (net.capacity, np.float64),
+            (net.fftime, np.float64), (net.B, np.float64), (net.power, np.float64),
+            (dem.start, np.int64), (dem.dest, np.int64), (dem.demand, np.float64))]
+        k = self._keep
+        self._net = _Network(net.num_zones, net.num_nodes, net.first_thru_node, net.num_links,
+                             *[a.ctypes.data for a in k[:6]])
+        self._dem = _Demand(dem.num_zones, *[a.ctypes.data for a in k[6:]])
+        self.E = int(net.num_links)
+        self.N = int(net.num_nodes)
+        self.Z = int(net.num_zones)
+        self.V = int(num_variants)
+        h = vp()
+        if num_variants == 1 and present is None and capacity is None and fftime is None \
+                and B is None and power is None:
+            rc = L.tapgpu_create(C.byref(h), C.byref(self._net), C.byref(self._dem))
+        else:
+            extra = []
+            for a, t in ((present, np.uint8), (capacity, np.float64), (fftime, np.float64),
+                         (B, np.float64), (power, np.float64)):
+                if a is None:
+                    extra.append(None)
+                else:
+                    arr = np.ascontiguousarray(a, dtype=t).reshape(-1)
+                    if arr.size != self.V * self.E:
+                        raise ValueError("per-variant arrays must be [num_variants, num_links]")
+                    extra.append(arr)
+            self._keep += [a for a in extra if a is not None]
+            rc = L.tapgpu_create_batched(C.byref(h), C.byref(self._net), C.byref(self._dem),
+                                         self.V, *[_p(a) for a in extra])
+        if rc != TAPGPU_OK:
+            raise TapGpuError(rc, L.tapgpu_last_error(None).decode())
+        self._h = h
+
+    # -- plumbing --
+    def _check(self, rc):
+        if rc != TAPGPU_OK:
+            msg = lib().tapgpu_last_error(self._h).decode()
+            raise (NoPathError if rc == -3 else TapGpuError)(rc, msg)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().tapgpu_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- shortest paths --
+    def set_costs(self, costs, variant=0):
+        c = np.ascontiguousarray(costs, dtype=np.float64)
+        assert c.size == self.E
+        self._check(lib().tapgpu_set_costs(self._h, variant, _p(c)))
+
+    def sssp(self, start_nodes, variant=0):
+        """(dist, pred) as [(N+1), len(start_nodes)] arrays (row = node id)."""
+        s = np.ascontiguousarray(start_nodes, dtype=np.int64)
+        n = s.size
+        d = np.empty((self.N + 1) * max(n, 1), dtype=np.float64)
+        p = np.empty((self.N + 1) * max(n, 1), dtype=np.int64)
+        self._check(lib().tapgpu_sssp(self._h, variant, _p(s), n, _p(d), _p(p), n, 0))
+        return d[:(self.N + 1) * n].reshape(self.N + 1, n), p[:(self.N + 1) * n].reshape(self.N + 1, n)
+
+    def aon(self, variant=0):
+        y = np.empty(self.E, dtype=np.float64)
+        self._check(lib().tapgpu_aon(self._h, variant, _p(y)))
+        return y
+
+    # -- Frank-Wolfe --
+    def fw_init(self, warm_volumes=None):
+        w = None
+        if warm_volumes is not None:
+            w = np.ascontiguousarray(warm_volumes, dtype=np.float64).reshape(-1)
+            assert w.size == self.V * self.E
+        self._check(lib().tapgpu_fw_init(self._h, _p(w)))
+
+    def fw_run(self, max_iters=0, target_relgap=0.0):
+        res = (_Result * self.V)()
+        self._check(lib().tapgpu_fw_run(self._h, int(max_iters), float(target_relgap),
+                                        C.cast(res, vp)))
+        return [dict(iterations=r.iterations, converged=bool(r.converged), relgap=r.relgap,
+                     objective=r.objective, vht=r.vht) for r in res]
+
+    def flows(self, variant=0):
+        x = np.empty(self.E, dtype=np.float64)
+        c = np.empty(self.E, dtype=np.float64)
+        self._check(lib().tapgpu_get_flows(self._h, variant, _p(x), _p(c)))
+        return x, c
+
+    def trace(self, variant=0):
+        n = C.c_int(0)
+        self._check(lib().tapgpu_get_trace(self._h, variant, None, 0, C.byref(n)))
+        buf = (_Iter * max(n.value, 1))()
+        self._check(lib().tapgpu_get_trace(self._h, variant, C.cast(buf, vp), n.value,
+                                           C.byref(n)))
+        return {f: np.array([getattr(buf[i], f) for i in range(n.value)]) for f in ITER_FIELDS}
+
+    def device_state(self):
+        x, c = vp(), vp()
+        self._check(lib().tapgpu_device_state(self._h, C.byref(x), C.byref(c)))
+        return x.value, c.value
+
+    def stats(self):
+        s = _Stats()
+        self._check(lib().tapgpu_get_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in _Stats._fields_}
