@@ -1,0 +1,175 @@
+"""Benchmark: TAP solves/sec to relative gap 1e-4 on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload metro]
+
+One step = one complete Frank-Wolfe solve (all-or-nothing start at free-flow
+costs, then iterations until relgap <= 1e-4) of the workload, with the
+network and demand already resident in HBM (uploaded before timing).
+Workloads: metro (default: jittered 100x100 grid + 1,000 zones, 43,600
+links), grid5 (24 nodes), regional (300x300 grid + 4,000 zones), batched
+(512 upgrade-subset variants of metro; value = variant solves/sec).
+
+Prints one JSON line with the throughput, the roofline of the dominant
+kernel (the shortest-path sweep) measured with HIP events around every
+sweep launch in the timed region, and the reference's CPU path timed on a
+bounded sample on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "TAP solves/sec to rel-gap 1e-4 (batched vs plain, 1 GPU); SSSP HBM GB/s"
+PEAK_HBM_GBS = 8000.0
+TARGET = 1e-4
+
+DESCR = {
+    "grid5": "grid5 (config 1): 24 nodes / 76 links / 24 zones",
+    "metro": "metro (config 2): 11,000 nodes / 43,600 links / 1,000 zones, gravity demand 400k",
+    "regional": "regional (config 3): 94,000 nodes / 386,800 links / 4,000 zones",
+}
+
+
+def sweep_bytes(st) -> float:
+    """Algorithmic bytes of the shortest-path sweeps (worklist form): per
+    processed (node, 64-origin group) item 8 B of frontier bookkeeping and
+    its own label row (64 x 8 B); per scanned in-link 12 B of link data and
+    the tail's label row (64 x 8 B); per lowered label 12 B (dist + pred);
+    per marked out-link 4 B."""
+    F, Es, U, Eo = st["sssp_items"], st["sssp_inlinks"], st["sssp_updates"], st["sssp_outlinks"]
+    return 8.0 * F + 12.0 * Es + 8.0 * 64 * (F + Es) + 12.0 * U + 4.0 * Eo
+
+
+def cpu_baseline(net, dem, gpu_iters: int, cores: int):
+    """The reference's tap_frankwolfe_pthread (compiled from its sources into
+    oracle/_ref) on this host: time 4 and 24 iterations, per-iteration time x
+    the GPU's iteration count + setup = one solve (extrapolated)."""
+    from traffic_assignment_amd import tntp
+    ref = os.path.join(ROOT, "oracle", "_ref", "tap_frankwolfe_pthread")
+    lo, hi = 4, 24
+    with tempfile.TemporaryDirectory() as td:
+        nf, tf = os.path.join(td, "net.txt"), os.path.join(td, "trips.txt")
+        tntp.write_net(nf, net)
+        tntp.write_trips(tf, dem)
+        if os.path.exists(ref):
+            kind = "reference"
+
+            def run(it):
+                t = time.perf_counter()
+                subprocess.run([ref, "-n", str(cores), "-i", str(it), nf, tf],
+                               stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, check=True)
+                return time.perf_counter() - t
+        else:
+            kind = "port"
+            import oracle
+
+            def run(it):
+                t = time.perf_counter()
+                oracle.fw_solve(net, dem, max_iters=it, nthreads=cores)
+                return time.perf_counter() - t
+        t_lo, t_hi = run(lo), run(hi)
+    per_iter = (t_hi - t_lo) / (hi - lo)
+    setup = max(t_lo - lo * per_iter, 0.0)
+    solve = setup + per_iter * gpu_iters
+    return {"value": 1.0 / solve, "unit": "solves/s", "cores": cores, "kind": kind,
+            "sample": (f"{DESCR.get(getattr(net, 'params', {}).get('workload', ''), 'workload')}: "
+                       f"{lo} and {hi} FW iterations timed, {per_iter * 1e3:.1f} ms/iteration, "
+                       f"extrapolated to the GPU's {gpu_iters} iterations + {setup:.2f} s setup"),
+            "seconds_per_iteration": per_iter, "measured_seconds": t_lo + t_hi}
+
+
+def load_traffic(workload):
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f).get(workload, {}).get("k_sssp_sweep_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="metro", choices=["grid5", "metro", "regional"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
+              "design is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    assert torch.cuda.is_available(), "no GPU"
+    torch.cuda.set_device(0)
+    from traffic_assignment_amd import synth
+    from traffic_assignment_amd.engine import TapEngine
+
+    net, dem = synth.make(args.workload)
+    eng = TapEngine(net, dem)          # graph, link data and demand -> HBM
+    for _ in range(args.warmup):
+        eng.fw_init()
+        eng.fw_run(target_relgap=TARGET)
+    eng.set_profiling(True)
+    acc = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters = []
+    for _ in range(args.steps):
+        eng.fw_init()
+        r = eng.fw_run(target_relgap=TARGET)[0]
+        iters.append(r["iterations"])
+        for k, v in eng.stats().items():
+            acc[k] = acc.get(k, 0) + v
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    eng.close()
+
+    value = args.steps / elapsed
+    launches = acc["sweep_launches"]
+    avg_ms = acc["sweep_kernel_ms"] / max(launches, 1)
+    bytes_per_launch = sweep_bytes(acc) / max(launches, 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    roof = {"bound": "hbm", "kernel": "k_sssp_sweep", "achieved": round(achieved, 2),
+            "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5),
+            "traffic": load_traffic(args.workload),
+            "avg_launch_us": round(avg_ms * 1e3, 3), "launches": int(launches),
+            "algorithmic_bytes_per_launch": round(bytes_per_launch, 1)}
+    cpu = None
+    if not args.no_cpu:
+        cores = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(net, dem, int(np.median(iters)), cores)
+    out = {
+        "metric": METRIC, "value": round(value, 5), "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded generator, traffic_assignment_amd/synth.py)",
+        "config": {"workload": DESCR[args.workload], "target_relgap": TARGET,
+                   "iterations": iters, "nodes": net.num_nodes, "links": net.num_links,
+                   "zones": net.num_zones, "parallelism": "1 GPU"},
+        "sssp_gbps": round(achieved, 2),
+        "phase_ms_per_step": {k: round(acc[k] / args.steps, 3) for k in
+                              ("sssp_ms", "aon_ms", "link_ms", "total_ms")},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_vs_cpu"] = round(value / cpu["value"], 2)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

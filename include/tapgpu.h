@@ -155,9 +155,22 @@ typedef struct {
   int64_t sssp_sweeps, aon_levels;             /* launches in the last fw_run  */
   int64_t sssp_items, aon_items;               /* (node, origin-group) visits  */
   int64_t link_passes;
+  int64_t sssp_inlinks;                        /* in-links scanned by sweeps   */
+  int64_t sssp_updates;                        /* labels lowered (lanes)       */
+  int64_t sssp_outlinks;                       /* out-links marked             */
+  int64_t aon_lanes;                           /* tree nodes loaded (lanes)    */
+  /* with profiling on: summed device time of the sweep / level launches,
+     measured with HIP events on the engine's stream around each launch */
+  double sweep_kernel_ms, level_kernel_ms;
+  int64_t sweep_launches, level_launches;
 } tapgpu_stats;
 
+/* Counters and wall times of the last tapgpu_fw_run (or of the calls since
+   the last fw_run for sssp/aon). */
 int tapgpu_get_stats(tapgpu_ctx *ctx, tapgpu_stats *stats);
+
+/* on != 0: time every sweep / level launch with HIP events (small overhead). */
+int tapgpu_set_profiling(tapgpu_ctx *ctx, int on);
 
 #ifdef __cplusplus
 }

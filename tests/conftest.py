@@ -1,0 +1,118 @@
+"""Shared fixtures.  `-m gpu` tests need an MI355X and the in-tree
+libtapgpu.so; everything else runs on the CPU against the oracle."""
+from __future__ import annotations
+
+import base64
+import functools
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an AMD GPU and the built HIP library")
+    config.addinivalue_line("markers", "slow: longer CPU test")
+
+
+@functools.lru_cache(maxsize=None)
+def network(name: str, scale: float = 1.0):
+    from traffic_assignment_amd import synth
+    return synth.make(name, scale=scale)
+
+
+@functools.lru_cache(maxsize=None)
+def golden(name: str):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return json.load(f)
+
+
+def b64_array(s: str, dtype) -> np.ndarray:
+    return np.frombuffer(base64.b64decode(s), dtype="<f8" if dtype == np.float64 else "<i8").copy()
+
+
+def hexarr(rows):
+    return np.array([[float.fromhex(v) for v in r] for r in rows])
+
+
+def sha(a: np.ndarray) -> str:
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a).astype(
+        "<f8" if a.dtype.kind == "f" else "<i8").tobytes()).hexdigest()
+
+
+def rmat(tag: str):
+    """(Network, 1-based) from a reference SSSP input graph fixture."""
+    from traffic_assignment_amd import synth
+    z = np.load(os.path.join(GOLDEN, f"rmat_{tag}.npz"))
+    fr, to, cost = z["frm"], z["to"], z["cost"]
+    n = int(max(fr.max(), to.max()))
+    order = np.lexsort((to, fr))
+    E = fr.size
+    net = synth.Network(num_zones=1, num_nodes=n, first_thru_node=1, init=fr[order].astype(np.int64),
+                        term=to[order].astype(np.int64), capacity=np.ones(E), length=np.ones(E),
+                        fftime=cost[order].astype(np.float64), B=np.zeros(E), power=np.ones(E),
+                        speed=np.zeros(E), toll=np.zeros(E), ltype=np.ones(E, np.int16))
+    return net
+
+
+def empty_demand(Z):
+    from traffic_assignment_amd import synth
+    return synth.Demand(Z, np.zeros(Z + 1, np.int64), np.zeros(0, np.int64), np.zeros(0))
+
+
+@pytest.fixture(scope="session")
+def oracle_lib():
+    import oracle
+    oracle.build()
+    oracle.lib().ora_set_pow_mode(0)
+    return oracle
+
+
+def gpu_available() -> bool:
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def engine_mod():
+    """The product's Python front end; on a GPU box a missing library fails."""
+    if not gpu_available():
+        pytest.skip("no GPU")
+    from traffic_assignment_amd import engine
+    engine.lib()
+    return engine
+
+
+def pred_tie_check(net, costs, dist, pred, o_pred, origins):
+    """Number of (node, origin) where pred differs from the oracle's and the
+    difference is NOT explained by an exact tie; and the number of ties.
+    dist/pred are [(N+1), len(origins)] arrays (row = node id)."""
+    ft = net.first_thru_node
+    term_sorted = np.argsort(net.term, kind="stable")
+    starts = np.searchsorted(net.term[term_sorted], np.arange(net.num_nodes + 2))
+    bad = ties = 0
+    diff = np.argwhere(pred != o_pred)
+    for v, j in diff:
+        if v == 0:
+            continue
+        src = origins[j]
+        ks = term_sorted[starts[v]:starts[v + 1]]
+        cand = [k for k in ks if (net.init[k] >= ft or net.init[k] == src)
+                and dist[net.init[k], j] < 1.7e308
+                and dist[net.init[k], j] + costs[k] == dist[v, j]]
+        if pred[v, j] in cand and o_pred[v, j] in cand:
+            ties += 1
+        else:
+            bad += 1
+    return bad, ties

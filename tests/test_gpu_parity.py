@@ -1,0 +1,198 @@
+"""Parity of the gfx950 engine (through the C ABI) with the oracle and the
+reference's golden vectors.  Bar: distances bit-exact, predecessor links
+equal wherever the shortest path is unique, loads exact (integer demand),
+the step-size sequence identical, volumes and costs bit-identical to the
+oracle with the correctly-rounded power (and within 1e-9 relative of the
+reference's glibc-pow run), relative gap / objective / VHT within 1e-9."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import b64_array, empty_demand, golden, hexarr, network, pred_tie_check, rmat
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-9
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=float); b = np.asarray(b, dtype=float)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))) if a.size else 0.0
+
+
+def _sssp_case(engine_mod, oracle_lib, net, dem, costs, origins):
+    with engine_mod.TapEngine(net, dem) as eng:
+        if costs is not net.fftime:
+            eng.set_costs(costs)
+        d, p = eng.sssp(origins)
+    od = np.empty_like(d); op = np.empty_like(p)
+    for j, s in enumerate(origins):
+        od[:, j], op[:, j] = oracle_lib.sssp(net, costs, int(s))
+    od[0] = np.finfo(float).max; op[0] = -1
+    assert np.array_equal(d.view(np.int64), od.view(np.int64))
+    bad, ties = pred_tie_check(net, costs, d, p, op, origins)
+    assert bad == 0, (bad, ties)
+    return ties
+
+
+@pytest.mark.parametrize("name,scale", [("grid5", 1.0), ("metro", 0.3)])
+def test_sssp_synthetic(engine_mod, oracle_lib, name, scale):
+    print(f"case sssp {name}@{scale}", flush=True)
+    net, dem = network(name, scale)
+    origins = np.arange(1, net.num_zones + 1)
+    _sssp_case(engine_mod, oracle_lib, net, dem, net.fftime, origins)
+    rng = np.random.default_rng(5)
+    costs = net.fftime * rng.uniform(1.0, 4.0, net.num_links)
+    _sssp_case(engine_mod, oracle_lib, net, dem, costs, origins)
+
+
+@pytest.mark.parametrize("tag", ["martin", "okuyama", "chicagosketch_flows",
+                                 "chicagoregional_flows"])
+def test_sssp_reference_graphs(engine_mod, oracle_lib, tag):
+    """The reference's own SSSP inputs (zero-cost links included)."""
+    print(f"case sssp rmat {tag}", flush=True)
+    net = rmat(tag)
+    n = net.num_nodes
+    origins = np.unique(np.r_[1, n, np.random.default_rng(9).integers(1, n + 1, 90)])
+    ties = _sssp_case(engine_mod, oracle_lib, net, empty_demand(1), net.fftime, origins)
+    print("equal-cost ties:", ties)
+
+
+def test_sssp_pape_cuda_layout(engine_mod, oracle_lib):
+    """Output layout of pape_cuda: [v * Z + s] with s the 1-based start node."""
+    import ctypes as C
+    net, dem = network("grid5")
+    Z = net.num_zones
+    with engine_mod.TapEngine(net, dem) as eng:
+        s = np.arange(1, Z + 1, dtype=np.int64)
+        d = np.full((net.num_nodes + 1) * (Z + 1), -7.0)
+        p = np.full((net.num_nodes + 1) * (Z + 1), -7, dtype=np.int64)
+        rc = engine_mod.lib().tapgpu_sssp(eng._h, 0, s.ctypes.data, Z, d.ctypes.data,
+                                          p.ctypes.data, Z, 1)
+        assert rc == 0
+    for o in range(1, Z + 1):
+        od, op = oracle_lib.sssp(net, net.fftime, o)
+        for v in range(1, net.num_nodes + 1):
+            assert d[v * Z + o] == od[v] and p[v * Z + o] == op[v]
+
+
+def test_sssp_unreachable_and_bad_args(engine_mod):
+    from traffic_assignment_amd import synth
+    # 1 -> 2, 3 isolated
+    net = synth.Network(num_zones=1, num_nodes=3, first_thru_node=1, init=np.array([1]),
+                        term=np.array([2]), capacity=np.ones(1), length=np.ones(1),
+                        fftime=np.array([1.5]), B=np.zeros(1), power=np.ones(1),
+                        speed=np.zeros(1), toll=np.zeros(1), ltype=np.ones(1, np.int16))
+    with engine_mod.TapEngine(net, empty_demand(1)) as eng:
+        d, p = eng.sssp([1, 3])
+        assert d[2, 0] == 1.5 and p[2, 0] == 0
+        assert d[3, 0] == np.finfo(float).max and p[3, 0] == -1
+        assert d[3, 1] == 0.0 and d[1, 1] == np.finfo(float).max
+        with pytest.raises(engine_mod.TapGpuError):
+            eng.sssp([4])
+
+
+@pytest.mark.parametrize("name,scale", [("grid5", 1.0), ("metro", 0.3), ("metro", 1.0)])
+def test_aon_exact(engine_mod, oracle_lib, name, scale):
+    print(f"case aon {name}@{scale}", flush=True)
+    net, dem = network(name, scale)
+    rng = np.random.default_rng(11)
+    costs = net.fftime * rng.uniform(1.0, 2.0, net.num_links)
+    with engine_mod.TapEngine(net, dem) as eng:
+        y0 = eng.aon()
+        eng.set_costs(costs)
+        y1 = eng.aon()
+    assert np.array_equal(y0, oracle_lib.fw_assign(net, dem, net.fftime, nthreads=8))
+    assert np.array_equal(y1, oracle_lib.fw_assign(net, dem, costs, nthreads=8))
+
+
+def test_aon_no_path_is_an_error(engine_mod):
+    from traffic_assignment_amd import synth
+    net = synth.Network(num_zones=2, num_nodes=2, first_thru_node=1, init=np.array([1]),
+                        term=np.array([1]), capacity=np.ones(1), length=np.ones(1),
+                        fftime=np.ones(1), B=np.zeros(1), power=np.ones(1), speed=np.zeros(1),
+                        toll=np.zeros(1), ltype=np.ones(1, np.int16))
+    dem = synth.Demand(2, np.array([0, 1, 1]), np.array([2]), np.array([5.0]))
+    with engine_mod.TapEngine(net, dem) as eng:
+        with pytest.raises(engine_mod.NoPathError, match="no path from 1 to 2"):
+            eng.aon()
+
+
+@pytest.mark.parametrize("name,scale,fix", [("grid5", 1.0, "grid5_fw.json"),
+                                            ("metro", 0.3, "metro_s03_fw.json"),
+                                            ("metro", 1.0, "metro_fw.json")])
+def test_fw_against_reference_golden(engine_mod, oracle_lib, name, scale, fix):
+    print(f"case fw {name}@{scale}", flush=True)
+    g = golden(fix)
+    net, dem = network(name, scale)
+    it = g["iterations"]
+    with engine_mod.TapEngine(net, dem) as eng:
+        eng.fw_init()
+        r = eng.fw_run(max_iters=it)[0]
+        x, c = eng.flows()
+        tr = eng.trace()
+    assert r["iterations"] == it
+    ref = hexarr(g["trace"])
+    fields = engine_mod.ITER_FIELDS
+    mine = np.stack([tr[f] for f in fields], axis=1)
+    assert np.array_equal(mine[:, 0], ref[:, 0])                  # step sizes
+    for j in range(1, len(fields)):
+        assert rel(mine[:, j], ref[:, j]) <= TOL, fields[j]
+    assert rel(r["vht"], float.fromhex(g["final_vht"])) <= TOL
+    if "x_b64" in g:
+        assert rel(x, b64_array(g["x_b64"], np.float64)) <= TOL
+        assert rel(c, b64_array(g["c_b64"], np.float64)) <= TOL
+    # bit-identical to the oracle when both round pow(x, 4) correctly
+    oracle_lib.lib().ora_set_pow_mode(1)
+    try:
+        o = oracle_lib.fw_solve(net, dem, max_iters=it, nthreads=8)
+    finally:
+        oracle_lib.lib().ora_set_pow_mode(0)
+    assert np.array_equal(tr["lambda_"], o["trace"]["lambda_"])
+    assert np.array_equal(x, o["x"]) and np.array_equal(c, o["c"])
+
+
+def test_fw_relgap_target_and_resume(engine_mod, oracle_lib):
+    """Stop rule (relgap > target is the loop test) and running in two
+    calls gives the same result as one call."""
+    net, dem = network("metro", 0.3)
+    with engine_mod.TapEngine(net, dem) as eng:
+        eng.fw_init()
+        r1 = eng.fw_run(target_relgap=1e-4)[0]
+        x1, _ = eng.flows()
+        eng.fw_init()
+        eng.fw_run(max_iters=5)
+        r2 = eng.fw_run(target_relgap=1e-4)[0]
+        x2, _ = eng.flows()
+        tr = eng.trace()
+    assert r1["converged"] and r1["relgap"] <= 1e-4
+    assert np.all(tr["relgap"][:-1] > 1e-4)
+    assert r1["iterations"] == r2["iterations"] and np.array_equal(x1, x2)
+    o = oracle_lib.fw_solve(net, dem, target_relgap=1e-4, nthreads=8)
+    assert o["iterations"] == r1["iterations"]
+
+
+def test_fw_warm_start_continues(engine_mod):
+    net, dem = network("metro", 0.3)
+    with engine_mod.TapEngine(net, dem) as eng:
+        eng.fw_init()
+        eng.fw_run(max_iters=10)
+        x10, _ = eng.flows()
+        eng.fw_run(max_iters=12)
+        x12, _ = eng.flows()
+        eng.fw_init(warm_volumes=x10)
+        eng.fw_run(max_iters=2)
+        xw, _ = eng.flows()
+    assert np.array_equal(x12, xw)
+
+
+def test_zero_demand_and_isolated(engine_mod):
+    net, _ = network("metro", 0.3)
+    dem = empty_demand(net.num_zones)
+    with engine_mod.TapEngine(net, dem) as eng:
+        assert np.all(eng.aon() == 0)
+        eng.fw_init()
+        r = eng.fw_run(max_iters=3)[0]
+        x, c = eng.flows()
+    assert np.all(x == 0) and np.array_equal(c, net.fftime)
+    assert r["iterations"] == 3

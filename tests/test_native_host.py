@@ -1,0 +1,53 @@
+"""CPU checks of the native code: the product's link arithmetic compiled for
+the host against the oracle, and the C ABI libraries' exported symbols
+(loaded without any GPU call)."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+BUILD = os.path.join(ROOT, "tests", "_build")
+
+
+def _compile_pow_check(oracle_lib):
+    os.makedirs(BUILD, exist_ok=True)
+    exe = os.path.join(BUILD, "pow_check")
+    src = os.path.join(ROOT, "tests", "native", "pow_check.c")
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-o", exe, src,
+                    os.path.join(ROOT, "oracle", "oracle.c"), "-lm", "-lpthread"], check=True)
+    return exe
+
+
+def test_product_pow4_bpr_integral_match_oracle_rounding(oracle_lib):
+    exe = _compile_pow_check(oracle_lib)
+    out = subprocess.run([exe, "2000000", "12345"], capture_output=True, text=True,
+                         check=True).stdout.split()
+    n, bad_oracle, bad_bpr, bad_int, bad_glibc = map(int, out)
+    assert n == 2000000
+    assert bad_oracle == 0 and bad_bpr == 0 and bad_int == 0
+    # glibc differs on a small fraction (its pow is not correctly rounded)
+    assert bad_glibc < n * 0.01
+
+
+def _declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(tapgpu_\w+|tap_frankwolfe|fw_assign|pape_cuda|"
+                                 r"link_volume_\w+|parse_\w+|print_flow|link_data_compar)\s*\(", txt)))
+
+
+def test_engine_library_exports_every_declared_symbol():
+    lib = os.path.join(ROOT, "traffic_assignment_amd", "libtapgpu.so")
+    if not os.path.exists(lib):
+        pytest.skip("libtapgpu.so not built")
+    L = ctypes.CDLL(lib)
+    names = _declared("tapgpu.h")
+    assert "tapgpu_fw_run" in names
+    for n in names:
+        assert hasattr(L, n), n

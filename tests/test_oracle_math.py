@@ -1,0 +1,190 @@
+"""Pins of the oracle that do not come from the reference's output: closed
+forms, brute force on tiny graphs, library shortest paths, invariants."""
+from __future__ import annotations
+
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from conftest import network, rmat
+
+
+# --- link functions ----------------------------------------------------------
+
+def test_bpr_closed_form(oracle_lib):
+    assert oracle_lib.bpr(2.0, 1000.0, 0.15, 4.0, 0.0) == 2.0
+    # Q = capacity: t0 * (1 + B)
+    assert oracle_lib.bpr(3.0, 800.0, 0.15, 4.0, 800.0) == 3.0 * 1.15
+    # Q = 2c: t0 * (1 + 16 B)
+    assert oracle_lib.bpr(1.0, 500.0, 0.15, 4.0, 1000.0) == 1.0 + 0.15 * 16.0
+
+
+def test_integral_is_antiderivative_of_bpr(oracle_lib):
+    t0, cap, B, p = 4.0, 2000.0, 0.15, 4.0
+    for Q in (0.0, 10.0, 1500.0, 4321.0):
+        # exact antiderivative t0*(Q + B*Q^5/(5 c^4)), and d/dQ == BPR
+        exact = t0 * (Q + B * Q ** 5 / (5 * cap ** 4))
+        got = oracle_lib.integral(t0, cap, B, p, Q)
+        assert got == pytest.approx(exact, rel=1e-14, abs=1e-300)
+        h = 1e-3
+        num = (oracle_lib.integral(t0, cap, B, p, Q + h) - oracle_lib.integral(t0, cap, B, p, max(Q - h, 0))) / (Q + h - max(Q - h, 0))
+        assert num == pytest.approx(oracle_lib.bpr(t0, cap, B, p, Q), rel=1e-6)
+    assert oracle_lib.integral(t0, cap, B, -1.0, 10.0) == 0.0   # power < 0 -> 0
+
+
+def test_correctly_rounded_pow4_mode(oracle_lib):
+    L = oracle_lib.lib()
+    import ctypes
+    L.ora_pow4_exact_check.restype = ctypes.c_double
+    L.ora_pow4_exact_check.argtypes = [ctypes.c_double]
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([rng.uniform(0, 5, 2000), 2.0 ** rng.uniform(-60, 60, 2000)])
+    for x in xs:
+        r = L.ora_pow4_exact_check(float(x))
+        ex = Fraction(float(x)) ** 4
+        for nb in (math.nextafter(r, -math.inf), math.nextafter(r, math.inf)):
+            assert abs(Fraction(r) - ex) <= abs(Fraction(nb) - ex)
+
+
+# --- line search ---------------------------------------------------------------
+
+def _two_link_net(t0=(1.0, 2.0), cap=(1000.0, 1000.0)):
+    from traffic_assignment_amd import synth
+    # nodes 1 -> 2 over two parallel links (1,2) twice is not allowed by
+    # sorting ambiguity; use 1->2 and 1->3->2 style: links (1,2), (1,3), (3,2)
+    init = np.array([1, 1, 3]); term = np.array([2, 3, 2])
+    return synth.Network(num_zones=2, num_nodes=3, first_thru_node=1, init=init, term=term,
+                         capacity=np.array([cap[0], cap[1], 1e12]), length=np.ones(3),
+                         fftime=np.array([t0[0], t0[1], 0.0]), B=np.array([0.15, 0.15, 0.0]),
+                         power=np.array([4.0, 4.0, 4.0]), speed=np.zeros(3), toll=np.zeros(3),
+                         ltype=np.ones(3, np.int16))
+
+
+def test_line_search_rule(oracle_lib):
+    net = _two_link_net()
+    # x: all 3000 on link 0; direction moves it to the other route
+    x = np.array([3000.0, 0.0, 0.0]); d = np.array([-3000.0, 3000.0, 3000.0])
+    lam = oracle_lib.line_search(net, x, d)
+    g = lambda l: oracle_lib.dirderiv(net, x, d, l)
+    # the rule: first 2^-k (k>=1) with g <= 0, or 0 / 1 at the ends
+    assert g(0) < 0 and g(1) > 0
+    k = round(-math.log2(lam))
+    assert lam == 2.0 ** -k and g(lam) <= 0 and g(2 * lam) > 0
+    # ends of the rule
+    assert oracle_lib.line_search(net, x, -d * 0) == 0.0          # g(0) = 0 >= 0
+    d2 = np.array([-10.0, 10.0, 10.0])
+    assert oracle_lib.line_search(net, x, d2) == 1.0               # g(1) <= 0
+
+
+# --- shortest paths ----------------------------------------------------------
+
+def _floyd(net, costs):
+    """All-pairs by Floyd-Warshall on exact sequential sums is not the same
+    arithmetic; instead brute force the least fixed point by repeated
+    relaxation to convergence (Bellman-Ford), in Python floats."""
+    N = net.num_nodes
+    INF = float("inf")
+    D = np.full((N + 1, N + 1), INF)
+    for s in range(1, N + 1):
+        d = [INF] * (N + 1)
+        d[s] = 0.0
+        changed = True
+        while changed:
+            changed = False
+            for k in range(net.num_links):
+                u, v = int(net.init[k]), int(net.term[k])
+                if d[u] == INF or (u < net.first_thru_node and u != s):
+                    continue
+                c = d[u] + float(costs[k])
+                if c < d[v]:
+                    d[v] = c; changed = True
+        D[s] = d
+    return D
+
+
+@pytest.mark.parametrize("tag", ["martin", "okuyama"])
+def test_sssp_tiny_graphs_brute_force(oracle_lib, tag):
+    net = rmat(tag)
+    D = _floyd(net, net.fftime)
+    for s in range(1, net.num_nodes + 1):
+        for lll in (True, False):
+            d, p = oracle_lib.sssp(net, net.fftime, s, lll=lll)
+            ref = np.where(np.isinf(D[s]), np.finfo(float).max, D[s])
+            assert np.array_equal(d[1:], ref[1:])
+
+
+def test_sssp_grid5_brute_force_and_zone_rule(oracle_lib):
+    net, dem = network("grid5")
+    D = _floyd(net, net.fftime)
+    for s in range(1, net.num_nodes + 1):
+        d, _ = oracle_lib.sssp(net, net.fftime, s)
+        assert np.array_equal(d[1:], D[s][1:])
+    # zone rule: metro zones are never inner nodes of a path
+    netm, _ = network("metro", 0.3)
+    d, p = oracle_lib.sssp(netm, netm.fftime, 1)
+    inner = netm.init[p[1:][p[1:] >= 0]]
+    assert np.all((inner >= netm.first_thru_node) | (inner == 1))
+
+
+@pytest.mark.parametrize("tag", ["chicagosketch_flows", "chicagoregional_flows"])
+def test_sssp_matches_scipy_dijkstra(oracle_lib, tag):
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import dijkstra
+    net = rmat(tag)
+    n = net.num_nodes
+    g = csr_matrix((net.fftime, (net.init, net.term)), shape=(n + 1, n + 1))
+    rng = np.random.default_rng(3)
+    srcs = rng.integers(1, n + 1, size=3)
+    D = dijkstra(g, directed=True, indices=srcs)
+    for i, s in enumerate(srcs):
+        d, _ = oracle_lib.sssp(net, net.fftime, int(s))
+        ref = np.where(np.isinf(D[i]), np.finfo(float).max, D[i])
+        # Dijkstra reaches the same least fixed point of fl(d[u] + w): the
+        # labels are bit-identical (these graphs include zero-cost links)
+        assert np.array_equal(d[1:], ref[1:])
+
+
+def test_sssp_fixed_point_invariants(oracle_lib):
+    net, _ = network("metro", 0.3)
+    rng = np.random.default_rng(1)
+    costs = net.fftime * rng.uniform(1.0, 3.0, size=net.num_links)
+    d, p = oracle_lib.sssp(net, costs, 7)
+    u = net.init; v = net.term
+    ok = (u >= net.first_thru_node) | (u == 7)
+    reach = d[u] < np.finfo(float).max
+    # no link can lower a label
+    assert np.all(d[v][ok & reach] <= d[u][ok & reach] + costs[ok & reach])
+    # every label is achieved by its predecessor link
+    has = p[1:] >= 0
+    k = p[1:][has]
+    assert np.array_equal(d[1:][has], d[net.init[k]] + costs[k])
+    assert np.array_equal(net.term[k], np.arange(1, net.num_nodes + 1)[has])
+
+
+# --- loading -------------------------------------------------------------------
+
+def test_aon_equals_brute_force_path_sums(oracle_lib):
+    net, dem = network("metro", 0.3)
+    y = oracle_lib.fw_assign(net, dem, net.fftime, nthreads=4)
+    ref = np.zeros(net.num_links)
+    for o in range(1, dem.num_zones + 1):
+        _, p = oracle_lib.sssp(net, net.fftime, o)
+        for i in range(dem.start[o - 1], dem.start[o]):
+            dst, f = int(dem.dest[i]), float(dem.demand[i])
+            if dst == o or f <= 0:
+                continue
+            v = dst
+            while v != o:
+                k = p[v]
+                ref[k] += f
+                v = net.init[k]
+    assert np.array_equal(y, ref)
+
+
+def test_aon_thread_count_does_not_change_integer_volumes(oracle_lib):
+    net, dem = network("metro", 0.3)
+    a = oracle_lib.fw_assign(net, dem, net.fftime, nthreads=1)
+    b = oracle_lib.fw_assign(net, dem, net.fftime, nthreads=7)
+    assert np.array_equal(a, b)

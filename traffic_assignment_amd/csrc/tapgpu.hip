@@ -88,42 +88,70 @@ struct Labels {
   const double* rvw;              // [V * E] link costs in reverse-CSR order
 };
 
-struct Queues {
-  const unsigned* qin;
-  unsigned* qout;
-  int* cnt;                       // per-sweep item counters
-  int* stamp;                     // [N * Galloc]
-};
-
 __device__ __forceinline__ bool group_on(const Labels& L, int g) {
   return L.var_on == nullptr || L.var_on[L.grp_var[g]] != 0;
 }
 
-// Queue the out-neighbours of v for group g (wave-cooperative).  Each (node,
-// group) enters a sweep's queue once: the first lane to swap the sweep's tag
-// into stamp[] appends it; appends are compacted with a ballot prefix.
-__device__ void enqueue_out(const Graph& G, const Queues& Q, int v, int g, int Galloc,
-                            int tag, int* cnt_next) {
+// Frontier bookkeeping: stamp[v * Galloc + g] holds the tag of the round in
+// which item (v, g) must be processed; flags[r] != 0 iff round r has work.
+// Writers only store (no atomics); a round's kernel finds its items with a
+// dense scan compacted per block in LDS.  An item whose stamp is advanced to
+// the next round before the scan reads it is simply processed one round
+// later, which both worklists tolerate (labels only decrease; loading lanes
+// carry the round from which they may run).
+struct Frontier {
+  int* stamp;     // [N * Galloc]
+  int* flags;     // [rounds]
+  unsigned long long* counters;   // [4] items, in-links scanned, lanes improved, out-links
+};
+
+// per-block scan of slots [blockIdx.x*kBlock, +kBlock) of the (v, g) space
+// of groups [g0, g0+Gn); returns the number of items compacted into s_items
+__device__ __forceinline__ int scan_slots(const Frontier& F, int N, int Galloc, int g0, int Gn,
+                                          int tag, unsigned* s_items, int* s_n) {
+  const long long slot = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const long long total = (long long)N * Gn;
+  bool act = false;
+  unsigned item = 0;
+  if (slot < total) {
+    const int v = (int)(slot / Gn);
+    const int g = g0 + (int)(slot - (long long)v * Gn);
+    item = (unsigned)v * (unsigned)Galloc + (unsigned)g;
+    act = F.stamp[item] == tag;
+  }
+  const unsigned long long m = ballot(act);
   const int lane = lane_id();
+  int base = 0;
+  if (lane == 0 && m) base = atomicAdd(s_n, (int)__popcll(m));
+  base = __shfl(base, 0);
+  if (act) s_items[base + (int)__popcll(m & ((1ull << lane) - 1ull))] = item;
+  __syncthreads();
+  return *s_n;
+}
+
+// Mark the out-neighbours of v for group g in the next round (wave-wide).
+__device__ __forceinline__ void mark_out(const Graph& G, const Frontier& F, int v, int g, int Galloc,
+                                         int tag_next) {
   const int beg = G.fw_off[v], end = G.fw_off[v + 1];
-  for (int base = beg; base < end; base += kWave) {
-    const int e = base + lane;
-    bool first = false;
-    int x = 0;
-    if (e < end) {
-      x = G.fw_dst[e];
-      const int old = atomicExch(&Q.stamp[(size_t)x * Galloc + g], tag);
-      first = (old != tag);
-    }
-    const unsigned long long m = ballot(first);
-    if (m) {
-      int pos = 0;
-      if (lane == 0) pos = atomicAdd(cnt_next, (int)__popcll(m));
-      pos = __shfl(pos, 0);
-      if (first) {
-        const int rank = (int)__popcll(m & ((1ull << lane) - 1ull));
-        Q.qout[pos + rank] = (unsigned)x * (unsigned)Galloc + (unsigned)g;
-      }
+  for (int e = beg + lane_id(); e < end; e += kWave)
+    F.stamp[(size_t)G.fw_dst[e] * Galloc + g] = tag_next;
+}
+
+__device__ __forceinline__ void block_flush(const Frontier& F, int* s_any, int* flag_next,
+                                            unsigned long long c0, unsigned long long c1,
+                                            unsigned long long c2, unsigned long long c3,
+                                            unsigned long long* s_cnt) {
+  // c0..c3 are per-wave totals held by lane 0
+  if (lane_id() == 0) {
+    atomicAdd(&s_cnt[0], c0); atomicAdd(&s_cnt[1], c1);
+    atomicAdd(&s_cnt[2], c2); atomicAdd(&s_cnt[3], c3);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (*s_any) *flag_next = 1;
+    if (s_cnt[0]) {
+      atomicAdd(&F.counters[0], s_cnt[0]); atomicAdd(&F.counters[1], s_cnt[1]);
+      atomicAdd(&F.counters[2], s_cnt[2]); atomicAdd(&F.counters[3], s_cnt[3]);
     }
   }
 }
@@ -148,36 +176,41 @@ __global__ __launch_bounds__(kBlock) void k_fill_labels(Graph G, Labels L, int g
   }
 }
 
-// Origins get label 0; their out-neighbours form the first sweep's queue.
-__global__ __launch_bounds__(kBlock) void k_sssp_seed(Graph G, Labels L, Queues Q, int g0,
-                                                       int Gn, int tag) {
+// Origins get label 0; their out-neighbours form round 0.
+__global__ __launch_bounds__(kBlock) void k_sssp_seed(Graph G, Labels L, Frontier F, int g0,
+                                                       int Gn, int tag0) {
   const int wave = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   if (wave >= Gn) return;
   const int g = g0 + wave;
   if (!group_on(L, g)) return;
   const int col = g * kWave + lane_id();
   const int src = L.col_src[col];
-  if (src < 0) return;
-  L.dist[(size_t)src * L.P + col] = 0.0;
-  for (int e = G.fw_off[src]; e < G.fw_off[src + 1]; ++e) {
-    const int x = G.fw_dst[e];
-    const int old = atomicExch(&Q.stamp[(size_t)x * L.Galloc + g], tag);
-    if (old != tag) {
-      const int pos = atomicAdd(&Q.cnt[0], 1);
-      Q.qout[pos] = (unsigned)x * (unsigned)L.Galloc + (unsigned)g;
+  bool any = false;
+  if (src >= 0) {
+    L.dist[(size_t)src * L.P + col] = 0.0;
+    for (int e = G.fw_off[src]; e < G.fw_off[src + 1]; ++e) {
+      F.stamp[(size_t)G.fw_dst[e] * L.Galloc + g] = tag0;
+      any = true;
     }
   }
+  if (ballot(any) && lane_id() == 0) F.flags[0] = 1;
 }
 
-// One sweep: every queued (v, g) pulls min over in-links of dist[u] + w for
-// its 64 origins (zones other than the lane's origin are not expanded).
-__global__ __launch_bounds__(kBlock) void k_sssp_sweep(Graph G, Labels L, Queues Q, int r,
-                                                        int tag) {
-  const int n = Q.cnt[r];
+// Round r: every marked (v, g) pulls min over in-links of dist[u] + w for its
+// 64 origins (zones other than the lane's origin are not expanded).
+__global__ __launch_bounds__(kBlock) void k_sssp_sweep(Graph G, Labels L, Frontier F, int g0,
+                                                        int Gn, int r, int tag, int tag_next) {
+  __shared__ unsigned s_items[kBlock];
+  __shared__ int s_n, s_any;
+  __shared__ unsigned long long s_cnt[4];
+  if (threadIdx.x < 4) s_cnt[threadIdx.x] = 0;
+  if (threadIdx.x == 0) { s_n = 0; s_any = 0; }
+  __syncthreads();
+  const int n = scan_slots(F, G.N, L.Galloc, g0, Gn, tag, s_items, &s_n);
   const int lane = lane_id();
-  const int waves = gridDim.x * kWavesPerBlock;
-  for (int it = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; it < n; it += waves) {
-    const unsigned item = (unsigned)uniform((int)Q.qin[it]);
+  unsigned long long c_items = 0, c_scan = 0, c_upd = 0, c_out = 0;
+  for (int it = threadIdx.x / kWave; it < n; it += kWavesPerBlock) {
+    const unsigned item = s_items[it];
     const int v = uniform((int)(item / (unsigned)L.Galloc));
     const int g = uniform((int)(item - (unsigned)v * (unsigned)L.Galloc));
     const int col = g * kWave + lane;
@@ -204,9 +237,17 @@ __global__ __launch_bounds__(kBlock) void k_sssp_sweep(Graph G, Labels L, Queues
       L.dist[me] = best;
       L.pred[me] = arg;
     }
-    if (ballot(improved && (v >= G.ft0 || v == src)))
-      enqueue_out(G, Q, v, g, L.Galloc, tag, &Q.cnt[r + 1]);
+    const unsigned long long m = ballot(improved);
+    c_items += 1;
+    c_scan += (unsigned long long)(ke - kb);
+    c_upd += (unsigned long long)__popcll(m);
+    if (ballot(improved && (v >= G.ft0 || v == src))) {
+      mark_out(G, F, v, g, L.Galloc, tag_next);
+      c_out += (unsigned long long)(G.fw_off[v + 1] - G.fw_off[v]);
+      if (lane == 0) s_any = 1;
+    }
   }
+  block_flush(F, &s_any, &F.flags[r + 1], c_items, c_scan, c_upd, c_out, s_cnt);
 }
 
 // ---------------------------------------------------------------------------
@@ -221,13 +262,18 @@ struct Loading {
   int* err;              // [3] flag, origin, dest (0-based)
 };
 
+// Lane state in pend[]: >= 0 children still to report; -1 done or not in the
+// tree; -(L+2) ready from round L on (its subtree flow is complete once all
+// launches before round L have finished).
+
 // Dense pass: children count, initial node flow (= demand to this zone),
-// no-path check, and the first ready set (leaves) as level 0.
-__global__ __launch_bounds__(kBlock) void k_aon_init(Graph G, Labels L, Loading A, Queues Q,
-                                                      int g0, int Gn) {
+// no-path check, leaves marked for round 0.
+__global__ __launch_bounds__(kBlock) void k_aon_init(Graph G, Labels L, Loading A, Frontier F,
+                                                      int g0, int Gn, int tag0) {
   const long long waves = (long long)gridDim.x * kWavesPerBlock;
   const long long items = (long long)G.N * Gn;
   const int lane = lane_id();
+  bool any = false;
   for (long long it = (long long)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; it < items;
        it += waves) {
     const int v = uniform((int)(it / Gn));
@@ -254,25 +300,31 @@ __global__ __launch_bounds__(kBlock) void k_aon_init(Graph G, Labels L, Loading 
       }
     }
     L.dist[me] = f;
-    // pend >= 0: children still to report; -1: done / not in the tree;
-    // -(r+2): ready to be processed in level r (leaves: level 0)
     L.pend[me] = (p < 0) ? -1 : (kids > 0 ? kids : -2);
-    if (ballot(p >= 0 && kids == 0) && lane == 0) {
-      const int pos = atomicAdd(&Q.cnt[0], 1);
-      Q.qout[pos] = (unsigned)v * (unsigned)L.Galloc + (unsigned)g;
+    if (ballot(p >= 0 && kids == 0)) {
+      if (lane == 0) F.stamp[(size_t)v * L.Galloc + g] = tag0;
+      any = true;
     }
   }
+  if (ballot(any) && lane == 0) F.flags[0] = 1;
 }
 
-// One level: ready lanes of each queued (v, g) add their subtree demand to
-// the in-link on their path, pass it to the tail node and release it.
-__global__ __launch_bounds__(kBlock) void k_aon_level(Graph G, Labels L, Loading A, Queues Q,
-                                                       int r, int tag) {
-  const int n = Q.cnt[r];
+// Round r: ready lanes of each marked (v, g) add their subtree demand to the
+// in-link on their path, pass it to the tail node and release it.
+__global__ __launch_bounds__(kBlock) void k_aon_level(Graph G, Labels L, Loading A, Frontier F,
+                                                       int g0, int Gn, int r, int tag,
+                                                       int tag_next) {
+  __shared__ unsigned s_items[kBlock];
+  __shared__ int s_n, s_any;
+  __shared__ unsigned long long s_cnt[4];
+  if (threadIdx.x < 4) s_cnt[threadIdx.x] = 0;
+  if (threadIdx.x == 0) { s_n = 0; s_any = 0; }
+  __syncthreads();
+  const int n = scan_slots(F, G.N, L.Galloc, g0, Gn, tag, s_items, &s_n);
   const int lane = lane_id();
-  const int waves = gridDim.x * kWavesPerBlock;
-  for (int it = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; it < n; it += waves) {
-    const unsigned item = (unsigned)uniform((int)Q.qin[it]);
+  unsigned long long c_items = 0, c_lanes = 0;
+  for (int it = threadIdx.x / kWave; it < n; it += kWavesPerBlock) {
+    const unsigned item = s_items[it];
     const int v = uniform((int)(item / (unsigned)L.Galloc));
     const int g = uniform((int)(item - (unsigned)v * (unsigned)L.Galloc));
     const int col = g * kWave + lane;
@@ -280,9 +332,9 @@ __global__ __launch_bounds__(kBlock) void k_aon_level(Graph G, Labels L, Loading
     const size_t me = (size_t)v * L.P + col;
     const int pe = L.pend[me];
     const int p = L.pred[me];
-    // only lanes released for this level: their children's flow additions
-    // were made in earlier launches and are visible
-    const bool ready = (pe == -(r + 2)) && (p >= 0);
+    // lanes released for this round or an earlier one (a lane released in
+    // this round carries -(r+3) and waits for the next launch)
+    const bool ready = (pe <= -2) && (-(pe + 2) <= r) && (p >= 0);
     const double f = ready ? L.dist[me] : 0.0;
     bool release = false;   // this lane's parent became ready
     if (ready) {
@@ -297,6 +349,8 @@ __global__ __launch_bounds__(kBlock) void k_aon_level(Graph G, Labels L, Loading
         }
       }
     }
+    c_items += 1;
+    c_lanes += (unsigned long long)__popcll(ballot(ready));
     // per in-link: wave sum of the flow routed through it (fixed order)
     double* __restrict__ yp = A.ypart + (size_t)g * G.E;
     for (int k = G.rv_off[v]; k < G.rv_off[v + 1]; ++k) {
@@ -308,19 +362,14 @@ __global__ __launch_bounds__(kBlock) void k_aon_level(Graph G, Labels L, Loading
       const bool rel = ballot(mine && release) != 0ull;
       if (lane == 0) {
         if (s != 0.0) yp[eid] += s;
-      }
-      if (rel) {
-        const int u = G.rv_src[k];
-        if (lane == 0) {
-          const int old = atomicExch(&Q.stamp[(size_t)u * L.Galloc + g], tag);
-          if (old != tag) {
-            const int pos = atomicAdd(&Q.cnt[r + 1], 1);
-            Q.qout[pos] = (unsigned)u * (unsigned)L.Galloc + (unsigned)g;
-          }
+        if (rel) {
+          F.stamp[(size_t)G.rv_src[k] * L.Galloc + g] = tag_next;
+          s_any = 1;
         }
       }
     }
   }
+  block_flush(F, &s_any, &F.flags[r + 1], c_items, c_lanes, 0, 0, s_cnt);
 }
 
 // y[var][e] = sum over the variant's groups of ypart (fixed order); clears
@@ -571,10 +620,11 @@ struct tapgpu_ctx {
   // labels
   double* dist = nullptr;
   int *pred = nullptr, *pend = nullptr, *col_src = nullptr, *grp_var = nullptr;
-  int *stamp = nullptr, *cnt = nullptr;
-  unsigned *qa = nullptr, *qb = nullptr;
-  int cnt_cap = 0;
+  int *stamp = nullptr, *flags = nullptr;
+  unsigned long long* counters = nullptr;
+  int flags_cap = 0;
   int tag = 0;
+  std::vector<int> zone_order;   // column -> zone (landmark order)
   // loading
   double *odc = nullptr, *ypart = nullptr;
   int* err_dev = nullptr;
@@ -589,6 +639,8 @@ struct tapgpu_ctx {
   int* h_int = nullptr;
   // stats
   tapgpu_stats stats{};
+  bool profiling = false;
+  hipEvent_t ev[2 * 8] = {};
   std::vector<void*> allocs;
 
   int fail(int code, const char* fmt, ...) {
@@ -658,35 +710,42 @@ Links make_links(tapgpu_ctx* c) {
                c->x, c->y, c->c, c->rvw};
 }
 
-// Worklist loop shared by the sweeps and the loading levels: launch
-// kCheckEvery rounds, then read the next round's queue length.
+// Round loop shared by the sweeps and the loading levels: launch
+// kCheckEvery rounds, then read whether the next round has work.
 template <class Launch>
-int run_rounds(tapgpu_ctx* c, Launch launch, int64_t* rounds_out, int64_t* items_out) {
+int run_rounds(tapgpu_ctx* c, Launch launch, int64_t* rounds_out, double* kernel_ms,
+               int64_t* launches) {
   int r = 0;
   for (;;) {
-    if (r + kCheckEvery + 2 >= c->cnt_cap)
+    if (r + kCheckEvery + 2 >= c->flags_cap)
       return c->fail(TAPGPU_E_LIMIT, "worklist did not drain in %d rounds", r);
     for (int k = 0; k < kCheckEvery; ++k, ++r) {
-      unsigned* qin = (r & 1) ? c->qb : c->qa;
-      unsigned* qout = (r & 1) ? c->qa : c->qb;
-      launch(r, qin, qout, c->tag + r + 1);
+      if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[2 * k], c->stream));
+      launch(r, c->tag + r, c->tag + r + 1);
+      if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[2 * k + 1], c->stream));
     }
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(c->h_int, c->cnt + r, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_int, c->flags + r - kCheckEvery + 1, sizeof(int) * kCheckEvery,
+                             hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (c->h_int[0] == 0) break;
-  }
-  if (items_out) {
-    std::vector<int> h(r + 1);
-    HIPCHK(c, hipMemcpy(h.data(), c->cnt, sizeof(int) * (r + 1), hipMemcpyDeviceToHost));
-    int64_t s = 0;
-    int last = 0;
-    for (int i = 0; i <= r; ++i) {
-      s += h[i];
-      if (h[i]) last = i + 1;
+    if (c->profiling) {
+      // rounds without work still launch; count every launch that was timed
+      for (int k = 0; k < kCheckEvery; ++k) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]));
+        *kernel_ms += ms;
+        *launches += 1;
+      }
     }
-    *items_out += s;
-    if (rounds_out) *rounds_out += last;
+    if (c->h_int[kCheckEvery - 1] == 0) {
+      if (rounds_out) {
+        int last = r - kCheckEvery;
+        for (int k = 0; k < kCheckEvery; ++k)
+          if (c->h_int[k]) last = r - kCheckEvery + 1 + k;
+        *rounds_out += last + 1;
+      }
+      break;
+    }
   }
   c->tag += r + 2;
   return TAPGPU_OK;
@@ -701,28 +760,47 @@ int maybe_reset_tags(tapgpu_ctx* c) {
   return TAPGPU_OK;
 }
 
+int scan_blocks(tapgpu_ctx* c, int Gn) {
+  return (int)(((long long)c->N * Gn + kBlock - 1) / kBlock);
+}
+
+int read_counters(tapgpu_ctx* c, int64_t* a, int64_t* b, int64_t* d, int64_t* e) {
+  unsigned long long h[4];
+  HIPCHK(c, hipMemcpyAsync(h, c->counters, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (a) *a += (int64_t)h[0];
+  if (b) *b += (int64_t)h[1];
+  if (d) *d += (int64_t)h[2];
+  if (e) *e += (int64_t)h[3];
+  return TAPGPU_OK;
+}
+
 // Shortest paths for groups [g0, g0+Gn).
 int sssp_groups(tapgpu_ctx* c, int g0, int Gn, const int* col_src, const int* grp_var,
                 const unsigned char* var_on) {
   int rc = maybe_reset_tags(c);
   if (rc) return rc;
+  auto t0 = std::chrono::steady_clock::now();
   Graph G = make_graph(c);
   Labels L = make_labels(c, col_src, grp_var, var_on);
-  HIPCHK(c, hipMemsetAsync(c->cnt, 0, sizeof(int) * c->cnt_cap, c->stream));
+  Frontier F{c->stamp, c->flags, c->counters};
+  HIPCHK(c, hipMemsetAsync(c->flags, 0, sizeof(int) * c->flags_cap, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 4, c->stream));
   hipLaunchKernelGGL(k_fill_labels, dim3(grid_for((long long)c->N * Gn)), dim3(kBlock), 0,
                      c->stream, G, L, g0, Gn);
-  Queues Q{c->qb, c->qa, c->cnt, c->stamp};
   hipLaunchKernelGGL(k_sssp_seed, dim3((Gn + kWavesPerBlock - 1) / kWavesPerBlock), dim3(kBlock), 0,
-                     c->stream, G, L, Q, g0, Gn, c->tag);
-  auto t0 = std::chrono::steady_clock::now();
+                     c->stream, G, L, F, g0, Gn, c->tag);
+  const int nb = scan_blocks(c, Gn);
   rc = run_rounds(
       c,
-      [&](int r, unsigned* qin, unsigned* qout, int tag) {
-        Queues q{qin, qout, c->cnt, c->stamp};
-        hipLaunchKernelGGL(k_sssp_sweep, dim3(kSweepBlocks), dim3(kBlock), 0, c->stream, G, L, q,
-                           r, tag);
+      [&](int r, int tag, int tag_next) {
+        hipLaunchKernelGGL(k_sssp_sweep, dim3(nb), dim3(kBlock), 0, c->stream, G, L, F, g0, Gn, r,
+                           tag, tag_next);
       },
-      &c->stats.sssp_sweeps, &c->stats.sssp_items);
+      &c->stats.sssp_sweeps, &c->stats.sweep_kernel_ms, &c->stats.sweep_launches);
+  if (rc) return rc;
+  rc = read_counters(c, &c->stats.sssp_items, &c->stats.sssp_inlinks, &c->stats.sssp_updates,
+                     &c->stats.sssp_outlinks);
   c->stats.sssp_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return rc;
 }
@@ -735,22 +813,22 @@ int aon_groups(tapgpu_ctx* c, int g0, int Gn, int v0, int Vn, const unsigned cha
   Graph G = make_graph(c);
   Labels L = make_labels(c, c->col_src, c->grp_var, var_on);
   Loading A{c->odc, c->Pv, c->Z, c->ypart, c->err_dev};
-  HIPCHK(c, hipMemsetAsync(c->cnt, 0, sizeof(int) * c->cnt_cap, c->stream));
+  Frontier F{c->stamp, c->flags, c->counters};
+  HIPCHK(c, hipMemsetAsync(c->flags, 0, sizeof(int) * c->flags_cap, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
-  Queues Q0{nullptr, c->qa, c->cnt, c->stamp};
   hipLaunchKernelGGL(k_aon_init, dim3(grid_for((long long)c->N * Gn)), dim3(kBlock), 0, c->stream,
-                     G, L, A, Q0, g0, Gn);
-  // level r reads qa when r is even (the init kernel wrote qa)
+                     G, L, A, F, g0, Gn, c->tag);
+  const int nb = scan_blocks(c, Gn);
   rc = run_rounds(
       c,
-      [&](int r, unsigned* qb_in, unsigned* qa_out, int tag) {
-        // run_rounds hands (qa, qb) for even r as (in=qa, out=qb)
-        Queues q{qb_in, qa_out, c->cnt, c->stamp};
-        hipLaunchKernelGGL(k_aon_level, dim3(kSweepBlocks), dim3(kBlock), 0, c->stream, G, L, A, q,
-                           r, tag);
+      [&](int r, int tag, int tag_next) {
+        hipLaunchKernelGGL(k_aon_level, dim3(nb), dim3(kBlock), 0, c->stream, G, L, A, F, g0, Gn,
+                           r, tag, tag_next);
       },
-      &c->stats.aon_levels, &c->stats.aon_items);
+      &c->stats.aon_levels, &c->stats.level_kernel_ms, &c->stats.level_launches);
   if (rc) return rc;
+  if ((rc = read_counters(c, &c->stats.aon_items, &c->stats.aon_lanes, nullptr, nullptr))) return rc;
   int herr[3];
   HIPCHK(c, hipMemcpyAsync(herr, c->err_dev, sizeof herr, hipMemcpyDeviceToHost, c->stream));
   hipLaunchKernelGGL(k_aon_reduce, dim3(grid_for((long long)c->E * Vn / kWave + 1)), dim3(kBlock), 0,
@@ -767,6 +845,77 @@ int launch_costs(tapgpu_ctx* c, int mode, const unsigned char* var_on, int v0, i
                      c->stream, make_links(c), mode, var_on, v0, Vn);
   HIPCHK(c, hipGetLastError());
   return TAPGPU_OK;
+}
+
+// Free-flow shortest-path distances from one node (binary-heap Dijkstra on
+// the host; zones other than the start are labelled but not expanded).
+std::vector<double> host_dijkstra(int N, int ft0, const std::vector<int>& off,
+                                  const std::vector<int>& dst, const std::vector<double>& w,
+                                  int s) {
+  std::vector<double> d(N, INFINITY);
+  std::vector<std::pair<double, int>> heap;
+  d[s] = 0.0;
+  heap.push_back({0.0, s});
+  auto cmp = [](const std::pair<double, int>& a, const std::pair<double, int>& b) {
+    return a.first > b.first;
+  };
+  while (!heap.empty()) {
+    std::pop_heap(heap.begin(), heap.end(), cmp);
+    const auto [du, u] = heap.back();
+    heap.pop_back();
+    if (du > d[u] || (u < ft0 && u != s)) continue;
+    for (int e = off[u]; e < off[u + 1]; ++e) {
+      const double nd = du + w[e];
+      if (nd < d[dst[e]]) {
+        d[dst[e]] = nd;
+        heap.push_back({nd, dst[e]});
+        std::push_heap(heap.begin(), heap.end(), cmp);
+      }
+    }
+  }
+  return d;
+}
+
+// Column order of the zones: origins that are close in the network share a
+// 64-lane group, so their wavefronts reach each node in nearby rounds.  The
+// "coordinates" of a zone are its free-flow distances from two far-apart
+// landmark nodes; zones are sorted along the Morton curve of those.  Results
+// do not depend on the order (each column is an independent search and the
+// group partial volumes are exact for integer demand).
+std::vector<int> landmark_zone_order(int N, int Z, int ft0, const std::vector<int>& off,
+                                     const std::vector<int>& dst, const std::vector<double>& w) {
+  std::vector<int> order(Z);
+  for (int z = 0; z < Z; ++z) order[z] = z;
+  if (Z <= kWave) return order;
+  auto farthest = [&](const std::vector<double>& d) {
+    int best = 0;
+    double bd = -1;
+    for (int v = 0; v < N; ++v)
+      if (std::isfinite(d[v]) && d[v] > bd) { bd = d[v]; best = v; }
+    return best;
+  };
+  const int l1 = farthest(host_dijkstra(N, ft0, off, dst, w, 0));
+  const std::vector<double> d1 = host_dijkstra(N, ft0, off, dst, w, l1);
+  const int l2 = farthest(d1);
+  const std::vector<double> d2 = host_dijkstra(N, ft0, off, dst, w, l2);
+  double m1 = 0, m2 = 0;
+  for (int z = 0; z < Z; ++z) {
+    if (std::isfinite(d1[z])) m1 = std::max(m1, d1[z]);
+    if (std::isfinite(d2[z])) m2 = std::max(m2, d2[z]);
+  }
+  auto q = [](double x, double m) -> uint32_t {
+    if (!std::isfinite(x) || m <= 0) return 0xFFFFu;
+    return (uint32_t)std::min(65535.0, std::floor(x / m * 65535.0));
+  };
+  auto spread = [](uint32_t x) -> uint64_t {
+    uint64_t r = 0;
+    for (int b = 0; b < 16; ++b) r |= (uint64_t)((x >> b) & 1u) << (2 * b);
+    return r;
+  };
+  std::vector<uint64_t> key(Z);
+  for (int z = 0; z < Z; ++z) key[z] = spread(q(d1[z], m1)) | (spread(q(d2[z], m2)) << 1);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+  return order;
 }
 
 int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, int64_t V,
@@ -824,6 +973,15 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
       rvpos[k] = pos;
     }
   }
+  // --- origin columns in landmark order ---
+  {
+    std::vector<double> w0(E);
+    for (int64_t k = 0; k < E; ++k)
+      w0[k] = (present && !present[k]) ? INFINITY : (t0_v ? t0_v[k] : net->free_flow_time[k]);
+    c->zone_order = landmark_zone_order(c->N, c->Z, c->ft0, fw_off, fw_dst, w0);
+  }
+  std::vector<int> col_of_zone(Z);
+  for (int64_t j = 0; j < Z; ++j) col_of_zone[c->zone_order[j]] = (int)j;
   // --- demand: dense per zone row in column order ---
   std::vector<double> odc((size_t)std::max<int64_t>(Z, 1) * c->Pv, 0.0);
   double tot = 0;
@@ -835,7 +993,7 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
       if (d == o || !(f > 0)) continue;
       if (d < 1 || d > Z)
         return c->fail(TAPGPU_E_ARG, "demand destination %lld is not a zone", (long long)d);
-      odc[(size_t)(d - 1) * c->Pv + (o - 1)] += f;
+      odc[(size_t)(d - 1) * c->Pv + col_of_zone[o - 1]] += f;
     }
   }
   c->total_demand = tot;
@@ -862,7 +1020,7 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
   }
   std::vector<int> col_src(c->P, -1), grp_var(c->Galloc);
   for (int v = 0; v < c->V; ++v) {
-    for (int64_t z = 0; z < Z; ++z) col_src[(size_t)v * c->Pv + z] = (int)z;
+    for (int64_t j = 0; j < Z; ++j) col_src[(size_t)v * c->Pv + j] = c->zone_order[j];
     for (int g = 0; g < c->Gv; ++g) grp_var[v * c->Gv + g] = v;
   }
 
@@ -880,9 +1038,9 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
   ALLOC(dist, NP); ALLOC(pred, NP); ALLOC(pend, NP);
   ALLOC(col_src, c->P); ALLOC(grp_var, c->Galloc);
   ALLOC(stamp, (size_t)N * c->Galloc);
-  ALLOC(qa, (size_t)N * c->Galloc); ALLOC(qb, (size_t)N * c->Galloc);
-  c->cnt_cap = (int)std::min<int64_t>(std::max<int64_t>(4 * N + 64, 1024), 1 << 24);
-  ALLOC(cnt, c->cnt_cap);
+  c->flags_cap = (int)std::min<int64_t>(std::max<int64_t>(4 * N + 64, 1024), 1 << 24);
+  ALLOC(flags, c->flags_cap);
+  ALLOC(counters, 4);
   ALLOC(odc, (size_t)std::max<int64_t>(Z, 1) * c->Pv);
   ALLOC(ypart, (size_t)c->Galloc * E);
   ALLOC(err_dev, 4);
@@ -957,6 +1115,8 @@ void tapgpu_destroy(tapgpu_ctx* c) {
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->trace) (void)hipFree(c->trace);
   if (c->h_int) (void)hipHostFree(c->h_int);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1228,6 +1388,14 @@ int tapgpu_device_state(tapgpu_ctx* c, const double** d_volumes, const double** 
 int tapgpu_get_stats(tapgpu_ctx* c, tapgpu_stats* s) {
   if (!c || !s) return TAPGPU_E_ARG;
   *s = c->stats;
+  return TAPGPU_OK;
+}
+
+int tapgpu_set_profiling(tapgpu_ctx* c, int on) {
+  if (!c) return TAPGPU_E_ARG;
+  if (on && !c->ev[0])
+    for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
+  c->profiling = on != 0;
   return TAPGPU_OK;
 }
 

@@ -57,7 +57,11 @@ class _Result(C.Structure):
 class _Stats(C.Structure):
     _fields_ = [("sssp_ms", C.c_double), ("aon_ms", C.c_double), ("link_ms", C.c_double),
                 ("total_ms", C.c_double), ("sssp_sweeps", C.c_int64), ("aon_levels", C.c_int64),
-                ("sssp_items", C.c_int64), ("aon_items", C.c_int64), ("link_passes", C.c_int64)]
+                ("sssp_items", C.c_int64), ("aon_items", C.c_int64), ("link_passes", C.c_int64),
+                ("sssp_inlinks", C.c_int64), ("sssp_updates", C.c_int64),
+                ("sssp_outlinks", C.c_int64), ("aon_lanes", C.c_int64),
+                ("sweep_kernel_ms", C.c_double), ("level_kernel_ms", C.c_double),
+                ("sweep_launches", C.c_int64), ("level_launches", C.c_int64)]
 
 
 _lib = None
@@ -88,6 +92,7 @@ def lib():
         L.tapgpu_get_trace.argtypes = [vp, C.c_int64, vp, C.c_int, C.POINTER(C.c_int)]
         L.tapgpu_device_state.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
         L.tapgpu_get_stats.argtypes = [vp, C.POINTER(_Stats)]
+        L.tapgpu_set_profiling.argtypes = [vp, C.c_int]
         _lib = L
     return _lib
 
@@ -217,6 +222,9 @@ class TapEngine:
         x, c = vp(), vp()
         self._check(lib().tapgpu_device_state(self._h, C.byref(x), C.byref(c)))
         return x.value, c.value
+
+    def set_profiling(self, on=True):
+        self._check(lib().tapgpu_set_profiling(self._h, int(bool(on))))
 
     def stats(self):
         s = _Stats()
