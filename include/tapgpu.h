@@ -105,7 +105,10 @@ int tapgpu_set_costs(tapgpu_ctx *ctx, int64_t variant, const double *costs);
    col_base + i for start node i.  With ld = num_start and col_base = 0
    this is a dense [(num_nodes+1) x num_start] matrix; pape_cuda's own
    layout is ld = num_start, col_base = 1 (column index = start node id when
-   start nodes are 1..num_start).  dist is DBL_MAX where unreachable; pred is
+   start nodes are 1..num_start; column num_start of row v is then the unused
+   column 0 of row v+1, as in the reference).  The buffers must hold
+   num_nodes * ld + col_base + num_start elements.  dist is DBL_MAX where
+   unreachable; pred is
    the link index of the last link of the path (-1: none).  Distances are
    bit-identical to the reference's CPU d'Esopo-Pape; pred matches wherever
    the shortest path is unique. */

@@ -1143,8 +1143,11 @@ int tapgpu_sssp(tapgpu_ctx* c, int64_t variant, const int64_t* start_nodes, int6
     return TAPGPU_E_ARG;
   int rc = check_variant(c, variant);
   if (rc) return rc;
-  if (ld < col_base + num_start)
-    return c->fail(TAPGPU_E_ARG, "ld %lld < col_base + num_start", (long long)ld);
+  // row v starts at v * ld; with pape_cuda's layout (ld = num_start,
+  // col_base = 1) the last column of row v is the unused column 0 of v + 1
+  if (ld < num_start || (col_base > 0 && ld < num_start + col_base - 1))
+    return c->fail(TAPGPU_E_ARG, "ld %lld too small for %lld columns", (long long)ld,
+                   (long long)num_start);
   for (int64_t i = 0; i < num_start; ++i)
     if (start_nodes[i] < 1 || start_nodes[i] > c->N)
       return c->fail(TAPGPU_E_ARG, "start node %lld out of range", (long long)start_nodes[i]);
