@@ -166,6 +166,10 @@ typedef struct {
      measured with HIP events on the engine's stream around each launch */
   double sweep_kernel_ms, level_kernel_ms;
   int64_t sweep_launches, level_launches;
+  /* one-origin-per-workgroup path: device time of its launches, launches,
+     and (origin, variant) workgroups run */
+  double origin_kernel_ms;
+  int64_t origin_launches, origin_blocks;
 } tapgpu_stats;
 
 /* Counters and wall times of the last tapgpu_fw_run (or of the calls since
@@ -174,6 +178,14 @@ int tapgpu_get_stats(tapgpu_ctx *ctx, tapgpu_stats *stats);
 
 /* on != 0: time every sweep / level launch with HIP events (small overhead). */
 int tapgpu_set_profiling(tapgpu_ctx *ctx, int on);
+
+/* Shortest-path / loading strategy: 0 = automatic (default), 1 = node-major
+   multi-origin sweeps (any size), 2 = one origin per workgroup with labels
+   in LDS (networks whose labels fit a workgroup's LDS: about 13k nodes,
+   fewer than 65535 links).  tapgpu_get_strategy returns the one in use
+   (1 or 2). */
+int tapgpu_set_strategy(tapgpu_ctx *ctx, int strategy);
+int tapgpu_get_strategy(const tapgpu_ctx *ctx);
 
 #ifdef __cplusplus
 }

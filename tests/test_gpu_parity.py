@@ -13,6 +13,16 @@ from conftest import b64_array, empty_demand, golden, hexarr, network, pred_tie_
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-9
+STRATS = ["sweeps", "lds"]
+
+
+def make_engine(engine_mod, net, dem, strategy, **kw):
+    eng = engine_mod.TapEngine(net, dem, **kw)
+    if strategy == "lds" and eng.strategy != "lds":
+        eng.close()
+        pytest.skip("network does not fit the LDS path")
+    eng.set_strategy(strategy)
+    return eng
 
 
 def rel(a, b):
@@ -20,8 +30,8 @@ def rel(a, b):
     return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))) if a.size else 0.0
 
 
-def _sssp_case(engine_mod, oracle_lib, net, dem, costs, origins):
-    with engine_mod.TapEngine(net, dem) as eng:
+def _sssp_case(engine_mod, oracle_lib, net, dem, costs, origins, strategy="sweeps"):
+    with make_engine(engine_mod, net, dem, strategy) as eng:
         if costs is not net.fftime:
             eng.set_costs(costs)
         d, p = eng.sssp(origins)
@@ -35,26 +45,28 @@ def _sssp_case(engine_mod, oracle_lib, net, dem, costs, origins):
     return ties
 
 
+@pytest.mark.parametrize("strategy", STRATS)
 @pytest.mark.parametrize("name,scale", [("grid5", 1.0), ("metro", 0.3)])
-def test_sssp_synthetic(engine_mod, oracle_lib, name, scale):
-    print(f"case sssp {name}@{scale}", flush=True)
+def test_sssp_synthetic(engine_mod, oracle_lib, name, scale, strategy):
+    print(f"case sssp {name}@{scale} {strategy}", flush=True)
     net, dem = network(name, scale)
     origins = np.arange(1, net.num_zones + 1)
-    _sssp_case(engine_mod, oracle_lib, net, dem, net.fftime, origins)
+    _sssp_case(engine_mod, oracle_lib, net, dem, net.fftime, origins, strategy)
     rng = np.random.default_rng(5)
     costs = net.fftime * rng.uniform(1.0, 4.0, net.num_links)
-    _sssp_case(engine_mod, oracle_lib, net, dem, costs, origins)
+    _sssp_case(engine_mod, oracle_lib, net, dem, costs, origins, strategy)
 
 
+@pytest.mark.parametrize("strategy", STRATS)
 @pytest.mark.parametrize("tag", ["martin", "okuyama", "chicagosketch_flows",
                                  "chicagoregional_flows"])
-def test_sssp_reference_graphs(engine_mod, oracle_lib, tag):
+def test_sssp_reference_graphs(engine_mod, oracle_lib, tag, strategy):
     """The reference's own SSSP inputs (zero-cost links included)."""
-    print(f"case sssp rmat {tag}", flush=True)
+    print(f"case sssp rmat {tag} {strategy}", flush=True)
     net = rmat(tag)
     n = net.num_nodes
     origins = np.unique(np.r_[1, n, np.random.default_rng(9).integers(1, n + 1, 90)])
-    ties = _sssp_case(engine_mod, oracle_lib, net, empty_demand(1), net.fftime, origins)
+    ties = _sssp_case(engine_mod, oracle_lib, net, empty_demand(1), net.fftime, origins, strategy)
     print("equal-cost ties:", ties)
 
 
@@ -92,13 +104,14 @@ def test_sssp_unreachable_and_bad_args(engine_mod):
             eng.sssp([4])
 
 
+@pytest.mark.parametrize("strategy", STRATS)
 @pytest.mark.parametrize("name,scale", [("grid5", 1.0), ("metro", 0.3), ("metro", 1.0)])
-def test_aon_exact(engine_mod, oracle_lib, name, scale):
-    print(f"case aon {name}@{scale}", flush=True)
+def test_aon_exact(engine_mod, oracle_lib, name, scale, strategy):
+    print(f"case aon {name}@{scale} {strategy}", flush=True)
     net, dem = network(name, scale)
     rng = np.random.default_rng(11)
     costs = net.fftime * rng.uniform(1.0, 2.0, net.num_links)
-    with engine_mod.TapEngine(net, dem) as eng:
+    with make_engine(engine_mod, net, dem, strategy) as eng:
         y0 = eng.aon()
         eng.set_costs(costs)
         y1 = eng.aon()
@@ -106,27 +119,29 @@ def test_aon_exact(engine_mod, oracle_lib, name, scale):
     assert np.array_equal(y1, oracle_lib.fw_assign(net, dem, costs, nthreads=8))
 
 
-def test_aon_no_path_is_an_error(engine_mod):
+@pytest.mark.parametrize("strategy", STRATS)
+def test_aon_no_path_is_an_error(engine_mod, strategy):
     from traffic_assignment_amd import synth
     net = synth.Network(num_zones=2, num_nodes=2, first_thru_node=1, init=np.array([1]),
                         term=np.array([1]), capacity=np.ones(1), length=np.ones(1),
                         fftime=np.ones(1), B=np.zeros(1), power=np.ones(1), speed=np.zeros(1),
                         toll=np.zeros(1), ltype=np.ones(1, np.int16))
     dem = synth.Demand(2, np.array([0, 1, 1]), np.array([2]), np.array([5.0]))
-    with engine_mod.TapEngine(net, dem) as eng:
+    with make_engine(engine_mod, net, dem, strategy) as eng:
         with pytest.raises(engine_mod.NoPathError, match="no path from 1 to 2"):
             eng.aon()
 
 
+@pytest.mark.parametrize("strategy", STRATS)
 @pytest.mark.parametrize("name,scale,fix", [("grid5", 1.0, "grid5_fw.json"),
                                             ("metro", 0.3, "metro_s03_fw.json"),
                                             ("metro", 1.0, "metro_fw.json")])
-def test_fw_against_reference_golden(engine_mod, oracle_lib, name, scale, fix):
-    print(f"case fw {name}@{scale}", flush=True)
+def test_fw_against_reference_golden(engine_mod, oracle_lib, name, scale, fix, strategy):
+    print(f"case fw {name}@{scale} {strategy}", flush=True)
     g = golden(fix)
     net, dem = network(name, scale)
     it = g["iterations"]
-    with engine_mod.TapEngine(net, dem) as eng:
+    with make_engine(engine_mod, net, dem, strategy) as eng:
         eng.fw_init()
         r = eng.fw_run(max_iters=it)[0]
         x, c = eng.flows()
@@ -152,11 +167,12 @@ def test_fw_against_reference_golden(engine_mod, oracle_lib, name, scale, fix):
     assert np.array_equal(x, o["x"]) and np.array_equal(c, o["c"])
 
 
-def test_fw_relgap_target_and_resume(engine_mod, oracle_lib):
+@pytest.mark.parametrize("strategy", STRATS)
+def test_fw_relgap_target_and_resume(engine_mod, oracle_lib, strategy):
     """Stop rule (relgap > target is the loop test) and running in two
     calls gives the same result as one call."""
     net, dem = network("metro", 0.3)
-    with engine_mod.TapEngine(net, dem) as eng:
+    with make_engine(engine_mod, net, dem, strategy) as eng:
         eng.fw_init()
         r1 = eng.fw_run(target_relgap=1e-4)[0]
         x1, _ = eng.flows()

@@ -217,25 +217,31 @@ __global__ __launch_bounds__(kBlock) void k_sssp_sweep(Graph G, Labels L, Fronti
     const int src = L.col_src[col];
     const double* __restrict__ w = L.rvw + (size_t)L.grp_var[g] * G.E;
     const size_t me = (size_t)v * L.P + col;
-    double best = L.dist[me];
+    const double cur = L.dist[me];
+    double best = cur, arg_du = INFINITY;
     int arg = -1;
     const int kb = G.rv_off[v], ke = G.rv_off[v + 1];
     for (int k = kb; k < ke; ++k) {
       const int u = G.rv_src[k];
-      if (u < G.ft0 && u != src) continue;          // zone: not a through node
+      if (u == v || (u < G.ft0 && u != src)) continue;   // zone: not a through node
       const double du = L.dist[(size_t)u * L.P + col];
       if (du < DBL_MAX) {
         const double cand = du + w[k];
-        if (cand < best) {
+        // equal-cost candidates: the tail with the smaller label wins (the
+        // one a label-ordered search expands first)
+        if (cand < best || (cand == best && du < arg_du)) {
           best = cand;
           arg = G.rv_eid[k];
+          arg_du = du;
         }
       }
     }
-    const bool improved = arg >= 0;
+    const bool improved = best < cur;
     if (improved) {
       L.dist[me] = best;
       L.pred[me] = arg;
+    } else if (arg >= 0 && arg_du < cur && L.pred[me] != arg) {
+      L.pred[me] = arg;     // same label, tail with a strictly smaller label
     }
     const unsigned long long m = ballot(improved);
     c_items += 1;
@@ -394,6 +400,242 @@ __global__ __launch_bounds__(kBlock) void k_aon_reduce(int E, int Gv, int v0, in
 }
 
 // ---------------------------------------------------------------------------
+// one origin per workgroup, labels in LDS (networks up to ~13k nodes)
+// ---------------------------------------------------------------------------
+//
+// For a metro-sized network one origin's labels (8 B/node) fit in a CU's
+// 160 KB LDS, so a 1024-thread workgroup runs the whole origin: rounds of
+// label-correcting relaxation (frontier bitmaps, 64-bit LDS atomicMin on the
+// label bits -- non-negative doubles order like their bit patterns), then
+// predecessor selection, then bottom-up loading of the tree with LDS FP64
+// atomics, adding link volumes to HBM with FP64 atomics.  One launch covers
+// every (variant, origin).
+//
+// Predecessor rule: among in-links u->v with fl(dist[u] + w) == dist[v]
+// (u expandable), take the tail with the smallest label (the one a
+// label-ordered search such as d'Esopo-Pape-LLL normally expands first),
+// then the smallest (round in which its label was last lowered, node id).
+// Along the chosen links (label, key) strictly decreases, so the tree is
+// acyclic even through zero-cost links; where the shortest path is unique
+// this is the reference's predecessor.
+
+constexpr int kLdsBlock = 1024;
+constexpr unsigned short kNoPred = 0xFFFF;
+
+struct OriginArgs {
+  int N, E, ft0, Z;
+  const int *fw_off, *fw_dst, *rv_off, *rv_src, *rv_eid, *tail;
+  const double* fww;                 // [V * E] forward costs (INF: absent)
+  const double* rvw;                 // [V * E] reverse-order costs
+  const long long* od_start;         // [Z + 1]
+  const int* od_dest;                // 0-based zone
+  const double* od_val;
+  const unsigned char* var_on;       // [V] or null
+  double* y;                         // [V * E], zeroed by the caller
+  int* err;                          // [3]
+  // shortest-path-only mode: block x handles start node src_list[x]
+  const int* src_list;
+  double* out_dist;                  // [(N) * ld] node-major, row v = node v+1
+  int* out_pred;
+  long long ld;
+};
+
+__device__ __forceinline__ double as_d(unsigned long long b) { return __longlong_as_double((long long)b); }
+__device__ __forceinline__ unsigned long long as_b(double d) { return (unsigned long long)__double_as_longlong(d); }
+
+__global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load) {
+  extern __shared__ unsigned long long sm[];
+  const int N = A.N;
+  const int W = (N + 63) >> 6;
+  unsigned long long* dist = sm;                                   // [N] (flow later)
+  unsigned long long* bm0 = dist + N;                              // [W]
+  unsigned long long* bm1 = bm0 + W;                               // [W]
+  unsigned short* pred = (unsigned short*)(bm1 + W);               // [N]
+  unsigned short* rnd = pred + ((N + 3) & ~3);                     // [N] (pend later)
+  unsigned* pendw = (unsigned*)rnd;                                // [ceil(N/4)]
+  __shared__ int s_more[2];
+
+  const int var = blockIdx.y;
+  if (A.var_on && !A.var_on[var]) return;
+  const int src = A.src_list ? A.src_list[blockIdx.x] : (int)blockIdx.x;
+  const int tid = threadIdx.x;
+  const double* __restrict__ fw = A.fww + (size_t)var * A.E;
+  const double* __restrict__ rw = A.rvw + (size_t)var * A.E;
+  const unsigned long long kMax = as_b(DBL_MAX);
+
+  for (int v = tid; v < N; v += kLdsBlock) { dist[v] = kMax; rnd[v] = 0; }
+  for (int w = tid; w < W; w += kLdsBlock) { bm0[w] = 0; bm1[w] = 0; }
+  if (tid == 0) { s_more[0] = 0; s_more[1] = 0; }
+  __syncthreads();
+  if (tid == 0) {
+    dist[src] = 0ull;                     // +0.0
+    bm0[src >> 6] = 1ull << (src & 63);
+  }
+  __syncthreads();
+
+  // ---- label-correcting rounds ----
+  unsigned long long* cur = bm0;
+  unsigned long long* nxt = bm1;
+  for (int r = 1;; ++r) {
+    for (int slot = tid; slot < 4 * W; slot += kLdsBlock) {
+      const int w = slot >> 2, q = slot & 3;
+      unsigned bits = (unsigned)((cur[w] >> (16 * q)) & 0xFFFFull);
+      while (bits) {
+        const int b = __ffs(bits) - 1;
+        bits &= bits - 1;
+        const int u = (w << 6) + 16 * q + b;
+        const double du = as_d(dist[u]);
+        const int eb = A.fw_off[u], ee = A.fw_off[u + 1];
+        for (int e = eb; e < ee; ++e) {
+          const int v = A.fw_dst[e];
+          const double cand = du + fw[e];
+          const unsigned long long cb = as_b(cand);
+          if (cb < dist[v]) {
+            const unsigned long long old = atomicMin(&dist[v], cb);
+            if (cb < old) {
+              rnd[v] = (unsigned short)r;
+              if (v >= A.ft0) {
+                atomicOr(&nxt[v >> 6], 1ull << (v & 63));
+                s_more[r & 1] = 1;
+              }
+            }
+          }
+        }
+      }
+    }
+    if (tid == 0) s_more[(r + 1) & 1] = 0;
+    __syncthreads();
+    for (int w = tid; w < W; w += kLdsBlock) cur[w] = 0;
+    const int more = s_more[r & 1];
+    __syncthreads();
+    if (!more) break;
+    if (r > N + 8) {            // only possible with negative costs
+      if (tid == 0 && atomicCAS(&A.err[0], 0, 2) == 0) { A.err[1] = src; A.err[2] = 0; }
+      return;
+    }
+    unsigned long long* t = cur; cur = nxt; nxt = t;
+  }
+
+  // ---- predecessor links ----
+  for (int v = tid; v < N; v += kLdsBlock) {
+    unsigned short p = kNoPred;
+    const unsigned long long dv = dist[v];
+    if (v != src && dv != kMax) {
+      unsigned best = 0xFFFFFFFFu;
+      unsigned long long best_du = ~0ull;
+      for (int k = A.rv_off[v]; k < A.rv_off[v + 1]; ++k) {
+        const int u = A.rv_src[k];
+        if (u == v || (u < A.ft0 && u != src)) continue;
+        const unsigned long long du = dist[u];
+        if (du == kMax) continue;
+        if (as_b(as_d(du) + rw[k]) != dv) continue;
+        const unsigned key = ((unsigned)rnd[u] << 16) | (unsigned)u;
+        if (du < best_du || (du == best_du && key < best)) {
+          best = key;
+          best_du = du;
+          p = (unsigned short)A.rv_eid[k];
+        }
+      }
+    }
+    pred[v] = p;
+    if (A.out_dist) {
+      const size_t o = (size_t)v * A.ld + blockIdx.x;
+      A.out_dist[o] = as_d(dv);
+      A.out_pred[o] = (p == kNoPred) ? -1 : (int)p;
+    }
+  }
+  __syncthreads();
+  if (!load) return;
+
+  // ---- all-or-nothing loading, bottom-up on the tree ----
+  double* flow = (double*)dist;
+  for (int v = tid; v < N; v += kLdsBlock) flow[v] = 0.0;
+  for (int i = tid; i < (N + 3) / 4; i += kLdsBlock) pendw[i] = 0u;
+  __syncthreads();
+  for (long long i = A.od_start[src] + tid; i < A.od_start[src + 1]; i += kLdsBlock) {
+    const int d = A.od_dest[i];
+    const double f = A.od_val[i];
+    if (d == src || !(f > 0.0)) continue;
+    atomicAdd(&flow[d], f);
+    if (pred[d] == kNoPred && atomicCAS(&A.err[0], 0, 1) == 0) { A.err[1] = src; A.err[2] = d; }
+  }
+  for (int v = tid; v < N; v += kLdsBlock) {
+    const unsigned short e = pred[v];
+    if (e == kNoPred) continue;
+    const int p = A.tail[e];
+    if (p != src) atomicAdd(&pendw[p >> 2], 1u << ((p & 3) * 8));
+  }
+  __syncthreads();
+  for (int w = tid; w < W; w += kLdsBlock) {
+    unsigned long long bits = 0;
+    for (int b = 0; b < 64; ++b) {
+      const int v = (w << 6) + b;
+      if (v >= N) break;
+      if (pred[v] != kNoPred && ((pendw[v >> 2] >> ((v & 3) * 8)) & 0xFFu) == 0u)
+        bits |= 1ull << b;
+    }
+    bm0[w] = bits;
+  }
+  __shared__ int s_tree, s_done;
+  if (tid == 0) { s_more[0] = 0; s_more[1] = 0; s_tree = 0; s_done = 0; }
+  __syncthreads();
+  {
+    int in_tree = 0;
+    for (int v = tid; v < N; v += kLdsBlock) in_tree += pred[v] != kNoPred;
+    atomicAdd(&s_tree, in_tree);
+  }
+  cur = bm0; nxt = bm1;
+  double* __restrict__ yv = A.y + (size_t)var * A.E;
+  int done = 0;
+  for (int r = 0;; ++r) {
+    for (int slot = tid; slot < 4 * W; slot += kLdsBlock) {
+      const int w = slot >> 2, q = slot & 3;
+      unsigned bits = (unsigned)((cur[w] >> (16 * q)) & 0xFFFFull);
+      while (bits) {
+        const int b = __ffs(bits) - 1;
+        bits &= bits - 1;
+        const int v = (w << 6) + 16 * q + b;
+        const int e = pred[v];
+        const int p = A.tail[e];
+        const double f = flow[v];
+        ++done;
+        if (f != 0.0) {
+          unsafeAtomicAdd(&yv[e], f);
+          if (p != src) atomicAdd(&flow[p], f);
+        }
+        if (p != src) {
+          const unsigned sh = (p & 3) * 8;
+          const unsigned old = atomicSub(&pendw[p >> 2], 1u << sh);
+          if (((old >> sh) & 0xFFu) == 1u) {
+            atomicOr(&nxt[p >> 6], 1ull << (p & 63));
+            s_more[r & 1] = 1;
+          }
+        }
+      }
+    }
+    if (tid == 0) s_more[(r + 1) & 1] = 0;
+    __syncthreads();
+    for (int w = tid; w < W; w += kLdsBlock) cur[w] = 0;
+    const int more = s_more[r & 1];
+    __syncthreads();
+    if (!more) break;
+    unsigned long long* t = cur; cur = nxt; nxt = t;
+  }
+  // every tree node must have been loaded (a predecessor cycle would not be)
+  atomicAdd(&s_done, done);
+  __syncthreads();
+  if (tid == 0 && s_done != s_tree && atomicCAS(&A.err[0], 0, 3) == 0) {
+    A.err[1] = src;
+    A.err[2] = 0;
+  }
+}
+
+size_t origin_lds_bytes(int N) {
+  const size_t W = (size_t)(N + 63) / 64;
+  return 8ull * N + 16ull * W + 2ull * ((N + 3) & ~3) + 2ull * ((N + 3) & ~3);
+}
+
+// ---------------------------------------------------------------------------
 // link passes
 // ---------------------------------------------------------------------------
 
@@ -403,7 +645,8 @@ struct Links {
   const unsigned char* present;      // [V * E]
   const int* rvpos;                  // [E] position of link k in reverse order
   double *x, *y, *c;                 // [V * E]
-  double* rvw;                       // [V * E]
+  double* rvw;                       // [V * E]  costs, reverse-CSR order, INF: absent
+  double* fww;                       // [V * E]  costs, link order, INF: absent
 };
 
 struct VarState {
@@ -426,7 +669,9 @@ __global__ __launch_bounds__(kBlock) void k_costs(Links K, int mode, const unsig
     else if (mode == 1) c = tap_bpr(K.t0[j], K.cap[j], K.B[j], K.pw[j], K.x[j]);
     else c = K.c[j];
     K.c[j] = c;
-    K.rvw[(size_t)var * K.E + K.rvpos[k]] = K.present[j] ? c : INFINITY;
+    const double wv = K.present[j] ? c : INFINITY;
+    K.rvw[(size_t)var * K.E + K.rvpos[k]] = wv;
+    K.fww[j] = wv;
   }
 }
 
@@ -572,7 +817,9 @@ __global__ __launch_bounds__(kBlock) void k_link_step(Links K, VarState* st) {
     K.x[i] = x;
     const double c = tap_bpr(K.t0[i], K.cap[i], K.B[i], K.pw[i], x);
     K.c[i] = c;
-    K.rvw[off + K.rvpos[k]] = K.present[i] ? c : INFINITY;
+    const double wv = K.present[i] ? c : INFINITY;
+    K.rvw[off + K.rvpos[k]] = wv;
+    K.fww[i] = wv;
   }
 }
 
@@ -616,7 +863,15 @@ struct tapgpu_ctx {
   // link records per variant
   double *t0 = nullptr, *cap = nullptr, *B = nullptr, *pw = nullptr;
   unsigned char* present = nullptr;
-  double *x = nullptr, *y = nullptr, *c = nullptr, *rvw = nullptr;
+  double *x = nullptr, *y = nullptr, *c = nullptr, *rvw = nullptr, *fww = nullptr;
+  // origin-major demand (per-origin workgroups)
+  long long* od_start = nullptr;
+  int* od_dest = nullptr;
+  double* od_val = nullptr;
+  // strategy: 0 auto, 1 node-major sweeps, 2 one origin per workgroup (LDS)
+  int strategy = 0;
+  bool lds_ok = false;
+  size_t lds_bytes = 0;
   // labels
   double* dist = nullptr;
   int *pred = nullptr, *pend = nullptr, *col_src = nullptr, *grp_var = nullptr;
@@ -707,7 +962,7 @@ Labels make_labels(tapgpu_ctx* c, const int* col_src, const int* grp_var,
 
 Links make_links(tapgpu_ctx* c) {
   return Links{c->E, c->V, c->t0, c->cap, c->B, c->pw, c->present, c->rvpos,
-               c->x, c->y, c->c, c->rvw};
+               c->x, c->y, c->c, c->rvw, c->fww};
 }
 
 // Round loop shared by the sweeps and the loading levels: launch
@@ -838,6 +1093,65 @@ int aon_groups(tapgpu_ctx* c, int g0, int Gn, int v0, int Vn, const unsigned cha
   if (herr[0])
     return c->fail(TAPGPU_E_NOPATH, "no path from %d to %d", herr[1] + 1, herr[2] + 1);
   return TAPGPU_OK;
+}
+
+bool use_lds(tapgpu_ctx* c) {
+  return c->lds_ok && (c->strategy == 0 || c->strategy == 2);
+}
+
+OriginArgs make_origin_args(tapgpu_ctx* c, const unsigned char* var_on) {
+  OriginArgs A{};
+  A.N = c->N; A.E = c->E; A.ft0 = c->ft0; A.Z = c->Z;
+  A.fw_off = c->fw_off; A.fw_dst = c->fw_dst; A.rv_off = c->rv_off; A.rv_src = c->rv_src;
+  A.rv_eid = c->rv_eid; A.tail = c->tail; A.fww = c->fww; A.rvw = c->rvw;
+  A.od_start = c->od_start; A.od_dest = c->od_dest; A.od_val = c->od_val;
+  A.var_on = var_on; A.y = c->y; A.err = c->err_dev;
+  return A;
+}
+
+// Shortest paths + loading for variants [v0, v0+Vn) with one workgroup per
+// (origin, variant); y of those variants is overwritten.
+int aon_lds(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
+  auto t0 = std::chrono::steady_clock::now();
+  HIPCHK(c, hipMemsetAsync(c->y + (size_t)v0 * c->E, 0, sizeof(double) * (size_t)Vn * c->E, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
+  if (c->Z > 0) {
+    OriginArgs A = make_origin_args(c, var_on);
+    A.fww += (size_t)v0 * c->E; A.rvw += (size_t)v0 * c->E; A.y += (size_t)v0 * c->E;
+    if (var_on) A.var_on = var_on + v0;
+    if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    hipLaunchKernelGGL(k_origin_lds, dim3(c->Z, Vn), dim3(kLdsBlock), c->lds_bytes, c->stream, A, 1);
+    if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(c, hipGetLastError());
+  }
+  int herr[3];
+  HIPCHK(c, hipMemcpyAsync(herr, c->err_dev, sizeof herr, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->profiling && c->Z > 0) {
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->stats.origin_kernel_ms += ms;
+    c->stats.origin_launches += 1;
+  }
+  c->stats.origin_blocks += (int64_t)c->Z * Vn;
+  c->stats.aon_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (herr[0] == 1)
+    return c->fail(TAPGPU_E_NOPATH, "no path from %d to %d", herr[1] + 1, herr[2] + 1);
+  if (herr[0] == 2)
+    return c->fail(TAPGPU_E_LIMIT, "label-correcting rounds exceeded from origin %d (negative costs?)",
+                   herr[1] + 1);
+  if (herr[0] == 3)
+    return c->fail(TAPGPU_E_LIMIT, "predecessor cycle in the tree of origin %d", herr[1] + 1);
+  return TAPGPU_OK;
+}
+
+// All-or-nothing loading of variants [v0, v0 + Vn) (groups of those
+// variants) with the current costs, by the configured strategy.
+int load_variants(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
+  if (use_lds(c)) return aon_lds(c, v0, Vn, var_on);
+  int rc = sssp_groups(c, v0 * c->Gv, Vn * c->Gv, c->col_src, c->grp_var, var_on);
+  if (!rc) rc = aon_groups(c, v0 * c->Gv, Vn * c->Gv, v0, Vn, var_on);
+  return rc;
 }
 
 int launch_costs(tapgpu_ctx* c, int mode, const unsigned char* var_on, int v0, int Vn) {
@@ -997,6 +1311,31 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
     }
   }
   c->total_demand = tot;
+  std::vector<long long> h_od_start(Z + 1, 0);
+  std::vector<int> h_od_dest;
+  std::vector<double> h_od_val;
+  for (int64_t o = 1; o <= Z; ++o) {
+    for (int64_t i = dem->od_start[o - 1]; i < dem->od_start[o]; ++i) {
+      const int64_t d = dem->od_dest[i];
+      const double f = dem->od_demand[i];
+      if (d == o || !(f > 0)) continue;
+      h_od_dest.push_back((int)(d - 1));
+      h_od_val.push_back(f);
+    }
+    h_od_start[o] = (long long)h_od_dest.size();
+  }
+  // one-origin-per-workgroup path: labels must fit LDS, links 16-bit,
+  // out-degree within the 8-bit child counters
+  {
+    int maxdeg = 0;
+    for (int64_t v = 0; v < N; ++v) maxdeg = std::max(maxdeg, fw_off[v + 1] - fw_off[v]);
+    int dev = 0, lds_max = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      lds_max = 0;
+    c->lds_bytes = origin_lds_bytes(c->N);
+    c->lds_ok = c->lds_bytes + 64 <= (size_t)lds_max && E < 65535 && maxdeg <= 255 && N < 65535;
+  }
 
   // --- link records per variant (capacity fix-up per variant) ---
   const size_t VE = (size_t)V * E;
@@ -1033,7 +1372,7 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
   ALLOC(fw_off, N + 1); ALLOC(fw_dst, E); ALLOC(rv_off, N + 1); ALLOC(rv_src, E);
   ALLOC(rv_eid, E); ALLOC(tail, E); ALLOC(rvpos, E);
   ALLOC(t0, VE); ALLOC(cap, VE); ALLOC(B, VE); ALLOC(pw, VE); ALLOC(present, VE);
-  ALLOC(x, VE); ALLOC(y, VE); ALLOC(c, VE); ALLOC(rvw, VE);
+  ALLOC(x, VE); ALLOC(y, VE); ALLOC(c, VE); ALLOC(rvw, VE); ALLOC(fww, VE);
   const size_t NP = (size_t)N * c->P;
   ALLOC(dist, NP); ALLOC(pred, NP); ALLOC(pend, NP);
   ALLOC(col_src, c->P); ALLOC(grp_var, c->Galloc);
@@ -1062,6 +1401,15 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
   if ((rc = upload(c, c->col_src, col_src.data(), c->P))) return rc;
   if ((rc = upload(c, c->grp_var, grp_var.data(), c->Galloc))) return rc;
   if ((rc = upload(c, c->odc, odc.data(), odc.size()))) return rc;
+  if ((rc = dalloc(c, &c->od_start, Z + 1))) return rc;
+  if ((rc = dalloc(c, &c->od_dest, h_od_dest.size()))) return rc;
+  if ((rc = dalloc(c, &c->od_val, h_od_val.size()))) return rc;
+  if ((rc = upload(c, c->od_start, h_od_start.data(), Z + 1))) return rc;
+  if ((rc = upload(c, c->od_dest, h_od_dest.data(), h_od_dest.size()))) return rc;
+  if ((rc = upload(c, c->od_val, h_od_val.data(), h_od_val.size()))) return rc;
+  if (c->lds_ok)
+    HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_origin_lds),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_bytes));
   HIPCHK(c, hipMemsetAsync(c->stamp, 0xFF, sizeof(int) * (size_t)N * c->Galloc, c->stream));
   HIPCHK(c, hipMemsetAsync(c->ypart, 0, sizeof(double) * (size_t)c->Galloc * E, c->stream));
   HIPCHK(c, hipMemsetAsync(c->x, 0, sizeof(double) * VE, c->stream));
@@ -1157,6 +1505,51 @@ int tapgpu_sssp(tapgpu_ctx* c, int64_t variant, const int64_t* start_nodes, int6
     if (dist) dist[col_base + i] = DBL_MAX;
     if (pred) pred[col_base + i] = -1;
   }
+  if (use_lds(c)) {
+    // one workgroup per start node; chunks of up to 1024 starts
+    const int64_t chunk = std::min<int64_t>(std::max<int64_t>(num_start, 1), 1024);
+    int* d_src = nullptr;
+    double* d_dist = nullptr;
+    int* d_pred = nullptr;
+    if ((rc = dalloc(c, &d_src, chunk))) return rc;
+    if ((rc = dalloc(c, &d_dist, (size_t)c->N * chunk))) return rc;
+    if ((rc = dalloc(c, &d_pred, (size_t)c->N * chunk))) return rc;
+    std::vector<int> hs(chunk);
+    std::vector<double> hd((size_t)c->N * chunk);
+    std::vector<int> hp((size_t)c->N * chunk);
+    for (int64_t s0 = 0; !rc && s0 < num_start; s0 += chunk) {
+      const int64_t ns = std::min<int64_t>(chunk, num_start - s0);
+      for (int64_t j = 0; j < ns; ++j) hs[j] = (int)(start_nodes[s0 + j] - 1);
+      if ((rc = upload(c, d_src, hs.data(), ns))) break;
+      OriginArgs A = make_origin_args(c, nullptr);
+      A.fww += (size_t)variant * c->E; A.rvw += (size_t)variant * c->E;
+      A.src_list = d_src; A.out_dist = d_dist; A.out_pred = d_pred; A.ld = chunk;
+      hipLaunchKernelGGL(k_origin_lds, dim3((unsigned)ns, 1), dim3(kLdsBlock), c->lds_bytes,
+                         c->stream, A, 0);
+      const hipError_t e0 = hipGetLastError();
+      const hipError_t e1 = hipMemcpyAsync(hd.data(), d_dist, sizeof(double) * c->N * chunk,
+                                           hipMemcpyDeviceToHost, c->stream);
+      const hipError_t e2 = hipMemcpyAsync(hp.data(), d_pred, sizeof(int) * c->N * chunk,
+                                           hipMemcpyDeviceToHost, c->stream);
+      const hipError_t e3 = hipStreamSynchronize(c->stream);
+      if (e0 != hipSuccess || e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+        rc = c->fail(TAPGPU_E_HIP, "shortest-path launch failed: %s", hipGetErrorString(
+                         e0 != hipSuccess ? e0 : (e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3))));
+        break;
+      }
+      for (int64_t v = 1; v < N1; ++v)
+        for (int64_t j = 0; j < ns; ++j) {
+          const size_t o = (size_t)v * ld + col_base + s0 + j;
+          if (dist) dist[o] = hd[(size_t)(v - 1) * chunk + j];
+          if (pred) pred[o] = hp[(size_t)(v - 1) * chunk + j];
+        }
+    }
+    for (void* q : {(void*)d_src, (void*)d_dist, (void*)d_pred}) {
+      c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), q), c->allocs.end());
+      (void)hipFree(q);
+    }
+    return rc;
+  }
   // chunks of the variant's columns; groups [0, Gv) of a private mapping
   int* d_col = nullptr;
   int* d_gv = nullptr;
@@ -1211,8 +1604,7 @@ int tapgpu_aon(tapgpu_ctx* c, int64_t variant, double* volumes) {
   unsigned char* d_on = nullptr;
   if ((rc = dalloc(c, &d_on, c->V))) return rc;
   rc = upload(c, d_on, on.data(), c->V);
-  if (!rc) rc = sssp_groups(c, v * c->Gv, c->Gv, c->col_src, c->grp_var, d_on);
-  if (!rc) rc = aon_groups(c, v * c->Gv, c->Gv, v, 1, d_on);
+  if (!rc) rc = load_variants(c, v, 1, d_on);
   if (!rc && volumes) {
     const hipError_t e = hipMemcpy(volumes, c->y + (size_t)v * c->E, sizeof(double) * c->E,
                                    hipMemcpyDeviceToHost);
@@ -1261,8 +1653,7 @@ int tapgpu_fw_init(tapgpu_ctx* c, const double* warm_volumes) {
     if ((rc = upload(c, c->x, warm_volumes, (size_t)c->V * c->E))) return rc;
   } else {
     if ((rc = launch_costs(c, 0, nullptr, 0, c->V))) return rc;   // Wa = t0
-    if ((rc = sssp_groups(c, 0, c->Galloc, c->col_src, c->grp_var, nullptr))) return rc;
-    if ((rc = aon_groups(c, 0, c->Galloc, 0, c->V, nullptr))) return rc;
+    if ((rc = load_variants(c, 0, c->V, nullptr))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->x, c->y, sizeof(double) * (size_t)c->V * c->E,
                              hipMemcpyDeviceToDevice, c->stream));
   }
@@ -1303,8 +1694,7 @@ int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_res
     }
     hipLaunchKernelGGL(k_begin_step, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->st, c->V,
                        c->var_on);
-    if ((rc = sssp_groups(c, 0, c->Galloc, c->col_src, c->grp_var, c->var_on))) return rc;
-    if ((rc = aon_groups(c, 0, c->Galloc, 0, c->V, c->var_on))) return rc;
+    if ((rc = load_variants(c, 0, c->V, c->var_on))) return rc;
     auto tl = std::chrono::steady_clock::now();
     hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kBlock), 0, c->stream, K, c->st, c->part, 0);
     hipLaunchKernelGGL(k_link_decide, dim3(c->V), dim3(kBlock), 0, c->stream, c->st, c->part,
@@ -1392,6 +1782,19 @@ int tapgpu_get_stats(tapgpu_ctx* c, tapgpu_stats* s) {
   if (!c || !s) return TAPGPU_E_ARG;
   *s = c->stats;
   return TAPGPU_OK;
+}
+
+int tapgpu_set_strategy(tapgpu_ctx* c, int strategy) {
+  if (!c || strategy < 0 || strategy > 2) return TAPGPU_E_ARG;
+  if (strategy == 2 && !c->lds_ok)
+    return c->fail(TAPGPU_E_ARG, "network does not fit the one-origin-per-workgroup path");
+  c->strategy = strategy;
+  return TAPGPU_OK;
+}
+
+int tapgpu_get_strategy(const tapgpu_ctx* c) {
+  if (!c) return TAPGPU_E_ARG;
+  return (c->lds_ok && c->strategy != 1) ? 2 : 1;
 }
 
 int tapgpu_set_profiling(tapgpu_ctx* c, int on) {

@@ -61,7 +61,9 @@ class _Stats(C.Structure):
                 ("sssp_inlinks", C.c_int64), ("sssp_updates", C.c_int64),
                 ("sssp_outlinks", C.c_int64), ("aon_lanes", C.c_int64),
                 ("sweep_kernel_ms", C.c_double), ("level_kernel_ms", C.c_double),
-                ("sweep_launches", C.c_int64), ("level_launches", C.c_int64)]
+                ("sweep_launches", C.c_int64), ("level_launches", C.c_int64),
+                ("origin_kernel_ms", C.c_double), ("origin_launches", C.c_int64),
+                ("origin_blocks", C.c_int64)]
 
 
 _lib = None
@@ -93,6 +95,8 @@ def lib():
         L.tapgpu_device_state.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
         L.tapgpu_get_stats.argtypes = [vp, C.POINTER(_Stats)]
         L.tapgpu_set_profiling.argtypes = [vp, C.c_int]
+        L.tapgpu_set_strategy.argtypes = [vp, C.c_int]
+        L.tapgpu_get_strategy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -222,6 +226,15 @@ class TapEngine:
         x, c = vp(), vp()
         self._check(lib().tapgpu_device_state(self._h, C.byref(x), C.byref(c)))
         return x.value, c.value
+
+    STRATEGIES = {"auto": 0, "sweeps": 1, "lds": 2}
+
+    def set_strategy(self, name):
+        self._check(lib().tapgpu_set_strategy(self._h, self.STRATEGIES[name]))
+
+    @property
+    def strategy(self):
+        return {1: "sweeps", 2: "lds"}[lib().tapgpu_get_strategy(self._h)]
 
     def set_profiling(self, on=True):
         self._check(lib().tapgpu_set_profiling(self._h, int(bool(on))))
