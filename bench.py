@@ -50,6 +50,21 @@ def sweep_bytes(st) -> float:
     return 8.0 * F + 12.0 * Es + 8.0 * 64 * (F + Es) + 12.0 * U + 4.0 * Eo
 
 
+def origin_bytes(st, num_links) -> float:
+    """Algorithmic bytes of the one-origin-per-workgroup kernel, summed over
+    its (origin, variant) workgroups: the same worklist formula with one
+    origin per row (8 B per expansion + its label, 12 B of link data and a
+    label per relaxed link, 12 B per lowered label), plus predecessor
+    selection (per link: 16 B of reverse-CSR data and the tail's label) and
+    loading (per tree node: 12 B of tree data, 8 B of flow, 16 B link-volume
+    read-modify-write)."""
+    F, Es, U = st["origin_expansions"], st["origin_scanned"], st["origin_updates"]
+    relax = 8.0 * F + 12.0 * Es + 8.0 * (F + Es) + 12.0 * U
+    pred = 24.0 * num_links * st["origin_blocks"]
+    load = 36.0 * st["origin_loaded"]
+    return relax + pred + load
+
+
 def cpu_baseline(net, dem, gpu_iters: int, cores: int):
     """The reference's tap_frankwolfe_pthread (compiled from its sources into
     oracle/_ref) on this host: time 4 and 24 iterations, per-iteration time x
@@ -94,7 +109,7 @@ def load_traffic(workload):
         return None
     try:
         with open(p) as f:
-            return json.load(f).get(workload, {}).get("k_sssp_sweep_bytes_per_launch")
+            return json.load(f).get(workload, {}).get("hbm_bytes_per_launch")
     except Exception:
         return None
 
@@ -106,6 +121,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="metro", choices=["grid5", "metro", "regional"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--strategy", default="auto", choices=["auto", "sweeps", "lds"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
@@ -120,6 +136,8 @@ def main():
 
     net, dem = synth.make(args.workload)
     eng = TapEngine(net, dem)          # graph, link data and demand -> HBM
+    if args.strategy != "auto":
+        eng.set_strategy(args.strategy)
     for _ in range(args.warmup):
         eng.fw_init()
         eng.fw_run(target_relgap=TARGET)
@@ -136,14 +154,22 @@ def main():
             acc[k] = acc.get(k, 0) + v
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    eng_path = eng.strategy
     eng.close()
 
     value = args.steps / elapsed
-    launches = acc["sweep_launches"]
-    avg_ms = acc["sweep_kernel_ms"] / max(launches, 1)
-    bytes_per_launch = sweep_bytes(acc) / max(launches, 1)
+    if acc.get("origin_launches", 0) > 0:
+        kernel = "k_origin_lds"
+        launches = acc["origin_launches"]
+        avg_ms = acc["origin_kernel_ms"] / launches
+        bytes_per_launch = origin_bytes(acc, net.num_links) / launches
+    else:
+        kernel = "k_sssp_sweep"
+        launches = acc["sweep_launches"]
+        avg_ms = acc["sweep_kernel_ms"] / max(launches, 1)
+        bytes_per_launch = sweep_bytes(acc) / max(launches, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    roof = {"bound": "hbm", "kernel": "k_sssp_sweep", "achieved": round(achieved, 2),
+    roof = {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2),
             "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5),
             "traffic": load_traffic(args.workload),
             "avg_launch_us": round(avg_ms * 1e3, 3), "launches": int(launches),
@@ -158,6 +184,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded generator, traffic_assignment_amd/synth.py)",
         "config": {"workload": DESCR[args.workload], "target_relgap": TARGET,
+                   "engine_path": eng_path,
                    "iterations": iters, "nodes": net.num_nodes, "links": net.num_links,
                    "zones": net.num_zones, "parallelism": "1 GPU"},
         "sssp_gbps": round(achieved, 2),

@@ -170,6 +170,11 @@ typedef struct {
      and (origin, variant) workgroups run */
   double origin_kernel_ms;
   int64_t origin_launches, origin_blocks;
+  int64_t origin_expansions;   /* node expansions (label-correcting)        */
+  int64_t origin_scanned;      /* out-links relaxed                         */
+  int64_t origin_updates;      /* labels lowered                            */
+  int64_t origin_relax_rounds, origin_load_rounds;  /* summed over origins  */
+  int64_t origin_loaded;       /* tree nodes loaded                         */
 } tapgpu_stats;
 
 /* Counters and wall times of the last tapgpu_fw_run (or of the calls since

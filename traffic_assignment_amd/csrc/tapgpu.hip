@@ -420,6 +420,8 @@ __global__ __launch_bounds__(kBlock) void k_aon_reduce(int E, int Gv, int v0, in
 // this is the reference's predecessor.
 
 constexpr int kLdsBlock = 1024;
+constexpr int kLdsMaxNodes = 16384;            // per-thread register arrays below
+constexpr int kLdsPerThread = kLdsMaxNodes / kLdsBlock;
 constexpr unsigned short kNoPred = 0xFFFF;
 
 struct OriginArgs {
@@ -433,6 +435,8 @@ struct OriginArgs {
   const unsigned char* var_on;       // [V] or null
   double* y;                         // [V * E], zeroed by the caller
   int* err;                          // [3]
+  unsigned long long* counters;      // [8] expansions, links scanned, updates,
+                                     //     relax rounds, load rounds, loaded nodes
   // shortest-path-only mode: block x handles start node src_list[x]
   const int* src_list;
   double* out_dist;                  // [(N) * ld] node-major, row v = node v+1
@@ -443,17 +447,29 @@ struct OriginArgs {
 __device__ __forceinline__ double as_d(unsigned long long b) { return __longlong_as_double((long long)b); }
 __device__ __forceinline__ unsigned long long as_b(double d) { return (unsigned long long)__double_as_longlong(d); }
 
+// LDS layout (bytes): labels 8N | three bitmaps 3*8W | offs/pred 2A |
+// rnd/ptail 2A | pend A, with W = ceil(N/64), A = N+1 rounded up to 4.
+size_t origin_lds_bytes(int N) {
+  const size_t W = (size_t)(N + 63) / 64, A = (size_t)((N + 1 + 3) & ~3);
+  return 8ull * N + 24ull * W + 5ull * A;
+}
+
 __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load) {
   extern __shared__ unsigned long long sm[];
   const int N = A.N;
   const int W = (N + 63) >> 6;
+  const int Al = (N + 1 + 3) & ~3;
   unsigned long long* dist = sm;                                   // [N] (flow later)
-  unsigned long long* bm0 = dist + N;                              // [W]
-  unsigned long long* bm1 = bm0 + W;                               // [W]
-  unsigned short* pred = (unsigned short*)(bm1 + W);               // [N]
-  unsigned short* rnd = pred + ((N + 3) & ~3);                     // [N] (pend later)
-  unsigned* pendw = (unsigned*)rnd;                                // [ceil(N/4)]
-  __shared__ int s_more[2];
+  unsigned long long* bmA = dist + N;                              // 3 x [W]
+  unsigned long long* bmB = bmA + W;
+  unsigned long long* bmC = bmB + W;
+  unsigned short* offs = (unsigned short*)(bmC + W);               // [N+1], then pred [N]
+  unsigned short* pred = offs;
+  unsigned short* rnd = offs + Al;                                 // [N], then ptail [N]
+  unsigned short* ptail = rnd;
+  unsigned* pendw = (unsigned*)(rnd + Al);                         // [A/4] packed bytes
+  __shared__ int s_more[3];
+  __shared__ unsigned long long s_cnt[8];
 
   const int var = blockIdx.y;
   if (A.var_on && !A.var_on[var]) return;
@@ -464,19 +480,23 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
   const unsigned long long kMax = as_b(DBL_MAX);
 
   for (int v = tid; v < N; v += kLdsBlock) { dist[v] = kMax; rnd[v] = 0; }
-  for (int w = tid; w < W; w += kLdsBlock) { bm0[w] = 0; bm1[w] = 0; }
-  if (tid == 0) { s_more[0] = 0; s_more[1] = 0; }
+  for (int v = tid; v <= N; v += kLdsBlock) offs[v] = (unsigned short)A.fw_off[v];
+  for (int w = tid; w < 3 * W; w += kLdsBlock) bmA[w] = 0;
+  if (tid < 3) s_more[tid] = 0;
+  if (tid < 8) s_cnt[tid] = 0;
   __syncthreads();
   if (tid == 0) {
     dist[src] = 0ull;                     // +0.0
-    bm0[src >> 6] = 1ull << (src & 63);
+    bmA[src >> 6] = 1ull << (src & 63);
   }
   __syncthreads();
 
-  // ---- label-correcting rounds ----
-  unsigned long long* cur = bm0;
-  unsigned long long* nxt = bm1;
-  for (int r = 1;; ++r) {
+  unsigned long long n_exp = 0, n_scan = 0, n_upd = 0;
+  // ---- label-correcting rounds (one barrier each: three rotating bitmaps,
+  //      the third one is cleared while the first is read) ----
+  unsigned long long *cur = bmA, *nxt = bmB, *clr = bmC;
+  int r = 1;
+  for (;; ++r) {
     for (int slot = tid; slot < 4 * W; slot += kLdsBlock) {
       const int w = slot >> 2, q = slot & 3;
       unsigned bits = (unsigned)((cur[w] >> (16 * q)) & 0xFFFFull);
@@ -485,7 +505,9 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
         bits &= bits - 1;
         const int u = (w << 6) + 16 * q + b;
         const double du = as_d(dist[u]);
-        const int eb = A.fw_off[u], ee = A.fw_off[u + 1];
+        const int eb = offs[u], ee = offs[u + 1];
+        ++n_exp;
+        n_scan += (unsigned long long)(ee - eb);
         for (int e = eb; e < ee; ++e) {
           const int v = A.fw_dst[e];
           const double cand = du + fw[e];
@@ -493,64 +515,88 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
           if (cb < dist[v]) {
             const unsigned long long old = atomicMin(&dist[v], cb);
             if (cb < old) {
+              ++n_upd;
               rnd[v] = (unsigned short)r;
               if (v >= A.ft0) {
                 atomicOr(&nxt[v >> 6], 1ull << (v & 63));
-                s_more[r & 1] = 1;
+                s_more[r % 3] = 1;
               }
             }
           }
         }
       }
     }
-    if (tid == 0) s_more[(r + 1) & 1] = 0;
+    for (int w = tid; w < W; w += kLdsBlock) clr[w] = 0;
+    if (tid == 0) s_more[(r + 1) % 3] = 0;
     __syncthreads();
-    for (int w = tid; w < W; w += kLdsBlock) cur[w] = 0;
-    const int more = s_more[r & 1];
-    __syncthreads();
-    if (!more) break;
+    if (!s_more[r % 3]) break;
     if (r > N + 8) {            // only possible with negative costs
       if (tid == 0 && atomicCAS(&A.err[0], 0, 2) == 0) { A.err[1] = src; A.err[2] = 0; }
       return;
     }
-    unsigned long long* t = cur; cur = nxt; nxt = t;
+    unsigned long long* t = cur; cur = nxt; nxt = clr; clr = t;
   }
+  const int relax_rounds = r;
 
-  // ---- predecessor links ----
-  for (int v = tid; v < N; v += kLdsBlock) {
-    unsigned short p = kNoPred;
-    const unsigned long long dv = dist[v];
-    if (v != src && dv != kMax) {
-      unsigned best = 0xFFFFFFFFu;
-      unsigned long long best_du = ~0ull;
-      for (int k = A.rv_off[v]; k < A.rv_off[v + 1]; ++k) {
-        const int u = A.rv_src[k];
-        if (u == v || (u < A.ft0 && u != src)) continue;
-        const unsigned long long du = dist[u];
-        if (du == kMax) continue;
-        if (as_b(as_d(du) + rw[k]) != dv) continue;
-        const unsigned key = ((unsigned)rnd[u] << 16) | (unsigned)u;
-        if (du < best_du || (du == best_du && key < best)) {
-          best = key;
-          best_du = du;
-          p = (unsigned short)A.rv_eid[k];
+  // ---- predecessor links (label, then round/node key; see above) ----
+  unsigned short my_tail[kLdsPerThread];
+  {
+    int j = 0;
+    for (int v = tid; v < N; v += kLdsBlock, ++j) {
+      unsigned short p = kNoPred, pt = 0;
+      const unsigned long long dv = dist[v];
+      if (v != src && dv != kMax) {
+        unsigned best = 0xFFFFFFFFu;
+        unsigned long long best_du = ~0ull;
+        for (int k = A.rv_off[v]; k < A.rv_off[v + 1]; ++k) {
+          const int u = A.rv_src[k];
+          if (u == v || (u < A.ft0 && u != src)) continue;
+          const unsigned long long du = dist[u];
+          if (du == kMax) continue;
+          if (as_b(as_d(du) + rw[k]) != dv) continue;
+          const unsigned key = ((unsigned)rnd[u] << 16) | (unsigned)u;
+          if (du < best_du || (du == best_du && key < best)) {
+            best = key;
+            best_du = du;
+            p = (unsigned short)A.rv_eid[k];
+            pt = (unsigned short)u;
+          }
         }
       }
-    }
-    pred[v] = p;
-    if (A.out_dist) {
-      const size_t o = (size_t)v * A.ld + blockIdx.x;
-      A.out_dist[o] = as_d(dv);
-      A.out_pred[o] = (p == kNoPred) ? -1 : (int)p;
+      my_tail[j] = pt;
+      if (A.out_dist) {
+        const size_t o = (size_t)v * A.ld + blockIdx.x;
+        A.out_dist[o] = as_d(dv);
+        A.out_pred[o] = (p == kNoPred) ? -1 : (int)p;
+      }
+      pred[v] = p;        // offs are no longer needed
     }
   }
-  __syncthreads();
-  if (!load) return;
+  __syncthreads();        // every rnd[] read is done
+  {
+    int j = 0;
+    for (int v = tid; v < N; v += kLdsBlock, ++j) ptail[v] = my_tail[j];
+  }
+  if (!load) {
+    if (A.counters) {
+      atomicAdd(&s_cnt[0], n_exp); atomicAdd(&s_cnt[1], n_scan); atomicAdd(&s_cnt[2], n_upd);
+      __syncthreads();
+      if (tid == 0) {
+        atomicAdd(&A.counters[0], s_cnt[0]); atomicAdd(&A.counters[1], s_cnt[1]);
+        atomicAdd(&A.counters[2], s_cnt[2]); atomicAdd(&A.counters[3], (unsigned long long)relax_rounds);
+      }
+    }
+    return;
+  }
 
   // ---- all-or-nothing loading, bottom-up on the tree ----
   double* flow = (double*)dist;
   for (int v = tid; v < N; v += kLdsBlock) flow[v] = 0.0;
-  for (int i = tid; i < (N + 3) / 4; i += kLdsBlock) pendw[i] = 0u;
+  for (int i = tid; i < Al / 4; i += kLdsBlock) pendw[i] = 0u;
+  for (int w = tid; w < 3 * W; w += kLdsBlock) bmA[w] = 0;
+  __shared__ int s_tree, s_done;
+  if (tid < 3) s_more[tid] = 0;
+  if (tid == 0) { s_tree = 0; s_done = 0; }
   __syncthreads();
   for (long long i = A.od_start[src] + tid; i < A.od_start[src + 1]; i += kLdsBlock) {
     const int d = A.od_dest[i];
@@ -559,12 +605,14 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
     atomicAdd(&flow[d], f);
     if (pred[d] == kNoPred && atomicCAS(&A.err[0], 0, 1) == 0) { A.err[1] = src; A.err[2] = d; }
   }
+  int in_tree = 0;
   for (int v = tid; v < N; v += kLdsBlock) {
-    const unsigned short e = pred[v];
-    if (e == kNoPred) continue;
-    const int p = A.tail[e];
+    if (pred[v] == kNoPred) continue;
+    ++in_tree;
+    const int p = ptail[v];
     if (p != src) atomicAdd(&pendw[p >> 2], 1u << ((p & 3) * 8));
   }
+  atomicAdd(&s_tree, in_tree);
   __syncthreads();
   for (int w = tid; w < W; w += kLdsBlock) {
     unsigned long long bits = 0;
@@ -574,20 +622,13 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
       if (pred[v] != kNoPred && ((pendw[v >> 2] >> ((v & 3) * 8)) & 0xFFu) == 0u)
         bits |= 1ull << b;
     }
-    bm0[w] = bits;
+    bmA[w] = bits;
   }
-  __shared__ int s_tree, s_done;
-  if (tid == 0) { s_more[0] = 0; s_more[1] = 0; s_tree = 0; s_done = 0; }
   __syncthreads();
-  {
-    int in_tree = 0;
-    for (int v = tid; v < N; v += kLdsBlock) in_tree += pred[v] != kNoPred;
-    atomicAdd(&s_tree, in_tree);
-  }
-  cur = bm0; nxt = bm1;
+  cur = bmA; nxt = bmB; clr = bmC;
   double* __restrict__ yv = A.y + (size_t)var * A.E;
   int done = 0;
-  for (int r = 0;; ++r) {
+  for (r = 1;; ++r) {
     for (int slot = tid; slot < 4 * W; slot += kLdsBlock) {
       const int w = slot >> 2, q = slot & 3;
       unsigned bits = (unsigned)((cur[w] >> (16 * q)) & 0xFFFFull);
@@ -595,12 +636,11 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
         const int b = __ffs(bits) - 1;
         bits &= bits - 1;
         const int v = (w << 6) + 16 * q + b;
-        const int e = pred[v];
-        const int p = A.tail[e];
+        const int p = ptail[v];
         const double f = flow[v];
         ++done;
         if (f != 0.0) {
-          unsafeAtomicAdd(&yv[e], f);
+          unsafeAtomicAdd(&yv[pred[v]], f);
           if (p != src) atomicAdd(&flow[p], f);
         }
         if (p != src) {
@@ -608,32 +648,33 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
           const unsigned old = atomicSub(&pendw[p >> 2], 1u << sh);
           if (((old >> sh) & 0xFFu) == 1u) {
             atomicOr(&nxt[p >> 6], 1ull << (p & 63));
-            s_more[r & 1] = 1;
+            s_more[r % 3] = 1;
           }
         }
       }
     }
-    if (tid == 0) s_more[(r + 1) & 1] = 0;
+    for (int w = tid; w < W; w += kLdsBlock) clr[w] = 0;
+    if (tid == 0) s_more[(r + 1) % 3] = 0;
     __syncthreads();
-    for (int w = tid; w < W; w += kLdsBlock) cur[w] = 0;
-    const int more = s_more[r & 1];
-    __syncthreads();
-    if (!more) break;
-    unsigned long long* t = cur; cur = nxt; nxt = t;
+    if (!s_more[r % 3]) break;
+    unsigned long long* t = cur; cur = nxt; nxt = clr; clr = t;
   }
   // every tree node must have been loaded (a predecessor cycle would not be)
   atomicAdd(&s_done, done);
+  if (A.counters) {
+    atomicAdd(&s_cnt[0], n_exp); atomicAdd(&s_cnt[1], n_scan); atomicAdd(&s_cnt[2], n_upd);
+  }
   __syncthreads();
-  if (tid == 0 && s_done != s_tree && atomicCAS(&A.err[0], 0, 3) == 0) {
-    A.err[1] = src;
-    A.err[2] = 0;
+  if (tid == 0) {
+    if (s_done != s_tree && atomicCAS(&A.err[0], 0, 3) == 0) { A.err[1] = src; A.err[2] = 0; }
+    if (A.counters) {
+      atomicAdd(&A.counters[0], s_cnt[0]); atomicAdd(&A.counters[1], s_cnt[1]);
+      atomicAdd(&A.counters[2], s_cnt[2]); atomicAdd(&A.counters[3], (unsigned long long)relax_rounds);
+      atomicAdd(&A.counters[4], (unsigned long long)r); atomicAdd(&A.counters[5], (unsigned long long)s_done);
+    }
   }
 }
 
-size_t origin_lds_bytes(int N) {
-  const size_t W = (size_t)(N + 63) / 64;
-  return 8ull * N + 16ull * W + 2ull * ((N + 3) & ~3) + 2ull * ((N + 3) & ~3);
-}
 
 // ---------------------------------------------------------------------------
 // link passes
@@ -1105,7 +1146,7 @@ OriginArgs make_origin_args(tapgpu_ctx* c, const unsigned char* var_on) {
   A.fw_off = c->fw_off; A.fw_dst = c->fw_dst; A.rv_off = c->rv_off; A.rv_src = c->rv_src;
   A.rv_eid = c->rv_eid; A.tail = c->tail; A.fww = c->fww; A.rvw = c->rvw;
   A.od_start = c->od_start; A.od_dest = c->od_dest; A.od_val = c->od_val;
-  A.var_on = var_on; A.y = c->y; A.err = c->err_dev;
+  A.var_on = var_on; A.y = c->y; A.err = c->err_dev; A.counters = c->counters;
   return A;
 }
 
@@ -1115,6 +1156,7 @@ int aon_lds(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
   auto t0 = std::chrono::steady_clock::now();
   HIPCHK(c, hipMemsetAsync(c->y + (size_t)v0 * c->E, 0, sizeof(double) * (size_t)Vn * c->E, c->stream));
   HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 8, c->stream));
   if (c->Z > 0) {
     OriginArgs A = make_origin_args(c, var_on);
     A.fww += (size_t)v0 * c->E; A.rvw += (size_t)v0 * c->E; A.y += (size_t)v0 * c->E;
@@ -1125,8 +1167,16 @@ int aon_lds(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
     HIPCHK(c, hipGetLastError());
   }
   int herr[3];
+  unsigned long long hc[8];
   HIPCHK(c, hipMemcpyAsync(herr, c->err_dev, sizeof herr, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(hc, c->counters, sizeof hc, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->stats.origin_expansions += (int64_t)hc[0];
+  c->stats.origin_scanned += (int64_t)hc[1];
+  c->stats.origin_updates += (int64_t)hc[2];
+  c->stats.origin_relax_rounds += (int64_t)hc[3];
+  c->stats.origin_load_rounds += (int64_t)hc[4];
+  c->stats.origin_loaded += (int64_t)hc[5];
   if (c->profiling && c->Z > 0) {
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
@@ -1334,7 +1384,8 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
     if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
       lds_max = 0;
     c->lds_bytes = origin_lds_bytes(c->N);
-    c->lds_ok = c->lds_bytes + 64 <= (size_t)lds_max && E < 65535 && maxdeg <= 255 && N < 65535;
+    c->lds_ok = c->lds_bytes + 128 <= (size_t)lds_max && E < 65535 && maxdeg <= 255 &&
+                N <= kLdsMaxNodes;
   }
 
   // --- link records per variant (capacity fix-up per variant) ---
@@ -1379,7 +1430,7 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
   ALLOC(stamp, (size_t)N * c->Galloc);
   c->flags_cap = (int)std::min<int64_t>(std::max<int64_t>(4 * N + 64, 1024), 1 << 24);
   ALLOC(flags, c->flags_cap);
-  ALLOC(counters, 4);
+  ALLOC(counters, 8);
   ALLOC(odc, (size_t)std::max<int64_t>(Z, 1) * c->Pv);
   ALLOC(ypart, (size_t)c->Galloc * E);
   ALLOC(err_dev, 4);

@@ -63,7 +63,10 @@ class _Stats(C.Structure):
                 ("sweep_kernel_ms", C.c_double), ("level_kernel_ms", C.c_double),
                 ("sweep_launches", C.c_int64), ("level_launches", C.c_int64),
                 ("origin_kernel_ms", C.c_double), ("origin_launches", C.c_int64),
-                ("origin_blocks", C.c_int64)]
+                ("origin_blocks", C.c_int64), ("origin_expansions", C.c_int64),
+                ("origin_scanned", C.c_int64), ("origin_updates", C.c_int64),
+                ("origin_relax_rounds", C.c_int64), ("origin_load_rounds", C.c_int64),
+                ("origin_loaded", C.c_int64)]
 
 
 _lib = None
