@@ -114,12 +114,89 @@ def load_traffic(workload):
         return None
 
 
+def bench_batched(args):
+    """Config 4: --variants upgrade subsets of the metro network solved to
+    relgap 1e-4 in one batch (one launch per FW step for all variants),
+    against the same engine solving a sample of the variants one at a time."""
+    import torch
+    from traffic_assignment_amd import variants as V
+    from traffic_assignment_amd.engine import TapEngine
+
+    base, dem, mods, subsets = V.workload(n_variants=args.variants)
+    eng, u, present = V.make_engine(base, dem, mods, subsets)
+    if args.strategy != "auto":
+        eng.set_strategy(args.strategy)
+    eng.set_profiling(True)
+    acc = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.fw_init()
+        res = eng.fw_run(target_relgap=TARGET)
+        for k, v in eng.stats().items():
+            acc[k] = acc.get(k, 0) + v
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    path = eng.strategy
+    eng.close()
+    iters = [r["iterations"] for r in res]
+    batched = args.steps * len(subsets) / elapsed
+    # plain: the same engine on single variant networks, one after another
+    sample = list(range(0, len(subsets), max(1, len(subsets) // args.plain_sample)))[:args.plain_sample]
+    plain_t = 0.0
+    for v in sample:
+        net = V.variant_network(base, mods, subsets[v])
+        with TapEngine(net, dem) as pe:
+            if args.strategy != "auto":
+                pe.set_strategy(args.strategy)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            pe.fw_init()
+            pe.fw_run(target_relgap=TARGET)
+            torch.cuda.synchronize()
+            plain_t += time.perf_counter() - t
+    plain = len(sample) / plain_t
+    launches = acc.get("origin_launches", 0)
+    avg_ms = acc.get("origin_kernel_ms", 0.0) / max(launches, 1)
+    bpl = origin_bytes(acc, u.num_links) / max(launches, 1)
+    achieved = bpl / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    cpu = None
+    if not args.no_cpu:
+        cores = min(16, os.cpu_count() or 1)
+        net0 = V.variant_network(base, mods, subsets[0])
+        c1 = cpu_baseline(net0, dem, int(np.median(iters)), cores)
+        cpu = dict(c1, sample="variant 0 of the batch: " + c1["sample"] + "; per-variant rate")
+    out = {
+        "metric": METRIC, "value": round(batched, 5), "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 0, "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded generator, traffic_assignment_amd/synth.py)",
+        "config": {"workload": f"batched (config 4): {len(subsets)} upgrade subsets of metro, "
+                               "32 candidate upgrades, budget 40", "target_relgap": TARGET,
+                   "engine_path": path, "iterations_min_median_max":
+                       [min(iters), int(np.median(iters)), max(iters)]},
+        "plain_solves_per_s": round(plain, 5), "batched_vs_plain": round(batched / plain, 3),
+        "roofline": {"bound": "hbm", "kernel": "k_origin_lds", "achieved": round(achieved, 2),
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5),
+                     "traffic": None, "avg_launch_us": round(avg_ms * 1e3, 3),
+                     "launches": int(launches), "algorithmic_bytes_per_launch": round(bpl, 1)},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_vs_cpu"] = round(batched / cpu["value"], 2)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="metro", choices=["grid5", "metro", "regional"])
+    ap.add_argument("--workload", default="metro",
+                    choices=["grid5", "metro", "regional", "batched"])
+    ap.add_argument("--variants", type=int, default=512, help="batched: number of variants")
+    ap.add_argument("--plain-sample", type=int, default=8,
+                    help="batched: variants solved one by one for the plain comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--strategy", default="auto", choices=["auto", "sweeps", "lds"])
     args = ap.parse_args()
@@ -134,6 +211,8 @@ def main():
     from traffic_assignment_amd import synth
     from traffic_assignment_amd.engine import TapEngine
 
+    if args.workload == "batched":
+        return bench_batched(args)
     net, dem = synth.make(args.workload)
     eng = TapEngine(net, dem)          # graph, link data and demand -> HBM
     if args.strategy != "auto":

@@ -1,0 +1,80 @@
+"""Batched upgrade-subset solves (one launch per Frank-Wolfe step for every
+variant) against plain solves of each variant network and the oracle."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from traffic_assignment_amd import synth, variants
+
+pytestmark = pytest.mark.gpu
+
+
+def _small():
+    base, dem = synth.metro(scale=0.3)
+    mods = synth.upgrades(base, n_change=6, n_add=4)
+    subsets = synth.budget_subsets(mods, n_variants=12, budget=25.0)
+    return base, dem, mods, subsets
+
+
+@pytest.mark.parametrize("strategy", ["sweeps", "lds"])
+def test_batched_equals_plain_and_oracle(engine_mod, oracle_lib, strategy):
+    print(f"case batched {strategy}", flush=True)
+    base, dem, mods, subsets = _small()
+    eng, u, present = variants.make_engine(base, dem, mods, subsets)
+    with eng:
+        eng.set_strategy(strategy)
+        eng.fw_init()
+        res = eng.fw_run(target_relgap=1e-4)
+        out = [(eng.flows(v), eng.trace(v)) for v in range(len(subsets))]
+    assert all(r["converged"] for r in res)
+    for v in (0, 3, 7, 11):
+        net = variants.variant_network(base, mods, subsets[v])
+        m = present[v].astype(bool)
+        with engine_mod.TapEngine(net, dem) as pe:
+            pe.set_strategy(strategy)
+            pe.fw_init()
+            pr = pe.fw_run(target_relgap=1e-4)[0]
+            px, pc = pe.flows()
+            ptr = pe.trace()
+        (bx, bc), btr = out[v]
+        assert pr["iterations"] == res[v]["iterations"]
+        assert np.array_equal(btr["lambda_"], ptr["lambda_"])
+        assert np.array_equal(bx[m], px) and np.all(bx[~m] == 0)
+        assert np.array_equal(bc[m], pc)
+        assert np.allclose(btr["relgap"], ptr["relgap"], rtol=1e-12, atol=0)
+        if v in (0, 7):
+            oracle_lib.lib().ora_set_pow_mode(1)
+            try:
+                o = oracle_lib.fw_solve(net, dem, target_relgap=1e-4, nthreads=8)
+            finally:
+                oracle_lib.lib().ora_set_pow_mode(0)
+            assert o["iterations"] == pr["iterations"]
+            assert np.array_equal(o["trace"]["lambda_"], ptr["lambda_"])
+            assert np.array_equal(o["x"], px)
+
+
+def test_batched_variants_stop_independently(engine_mod):
+    base, dem, mods, subsets = _small()
+    eng, u, present = variants.make_engine(base, dem, mods, subsets)
+    with eng:
+        eng.fw_init()
+        res = eng.fw_run(max_iters=50, target_relgap=1e-4)
+        iters = [r["iterations"] for r in res]
+        x_before = [eng.flows(v)[0] for v in range(len(subsets))]
+        res2 = eng.fw_run(max_iters=50, target_relgap=1e-4)
+        x_after = [eng.flows(v)[0] for v in range(len(subsets))]
+    # converged variants are frozen: a second run changes nothing
+    assert all(r["converged"] for r in res)
+    assert [r["iterations"] for r in res2] == iters
+    assert all(np.array_equal(a, b) for a, b in zip(x_before, x_after))
+    # an iteration cap stops each variant at the cap; continuing resumes it
+    eng2, _, _ = variants.make_engine(base, dem, mods, subsets)
+    with eng2:
+        eng2.fw_init()
+        capped = eng2.fw_run(max_iters=5)
+        assert all(r["iterations"] == 5 for r in capped)
+        done = eng2.fw_run(target_relgap=1e-4)
+        x_resumed = [eng2.flows(v)[0] for v in range(len(subsets))]
+    assert [r["iterations"] for r in done] == iters
+    assert all(np.array_equal(a, b) for a, b in zip(x_before, x_resumed))

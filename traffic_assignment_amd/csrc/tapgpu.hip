@@ -911,6 +911,8 @@ struct tapgpu_ctx {
   double* od_val = nullptr;
   // strategy: 0 auto, 1 node-major sweeps, 2 one origin per workgroup (LDS)
   int strategy = 0;
+  bool labels_ready = false;          // node-major buffers allocated
+  std::vector<double> h_odc;          // demand by (zone row, column), host copy
   bool lds_ok = false;
   size_t lds_bytes = 0;
   // labels
@@ -1071,10 +1073,31 @@ int read_counters(tapgpu_ctx* c, int64_t* a, int64_t* b, int64_t* d, int64_t* e)
   return TAPGPU_OK;
 }
 
+// Node-major buffers of the multi-origin sweep path, allocated on first use
+// (the one-origin-per-workgroup path never needs them).
+int ensure_labels(tapgpu_ctx* c) {
+  if (c->labels_ready) return TAPGPU_OK;
+  const size_t NP = (size_t)c->N * c->P;
+  int rc;
+  if ((rc = dalloc(c, &c->dist, NP))) return rc;
+  if ((rc = dalloc(c, &c->pred, NP))) return rc;
+  if ((rc = dalloc(c, &c->pend, NP))) return rc;
+  if ((rc = dalloc(c, &c->stamp, (size_t)c->N * c->Galloc))) return rc;
+  if ((rc = dalloc(c, &c->odc, c->h_odc.size()))) return rc;
+  if ((rc = dalloc(c, &c->ypart, (size_t)c->Galloc * c->E))) return rc;
+  if ((rc = upload(c, c->odc, c->h_odc.data(), c->h_odc.size()))) return rc;
+  HIPCHK(c, hipMemsetAsync(c->stamp, 0xFF, sizeof(int) * (size_t)c->N * c->Galloc, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->ypart, 0, sizeof(double) * (size_t)c->Galloc * c->E, c->stream));
+  c->labels_ready = true;
+  return TAPGPU_OK;
+}
+
 // Shortest paths for groups [g0, g0+Gn).
 int sssp_groups(tapgpu_ctx* c, int g0, int Gn, const int* col_src, const int* grp_var,
                 const unsigned char* var_on) {
-  int rc = maybe_reset_tags(c);
+  int rc = ensure_labels(c);
+  if (rc) return rc;
+  rc = maybe_reset_tags(c);
   if (rc) return rc;
   auto t0 = std::chrono::steady_clock::now();
   Graph G = make_graph(c);
@@ -1103,7 +1126,9 @@ int sssp_groups(tapgpu_ctx* c, int g0, int Gn, const int* col_src, const int* gr
 
 // Loading for groups [g0, g0+Gn) of variants [v0, v0+Vn) into y.
 int aon_groups(tapgpu_ctx* c, int g0, int Gn, int v0, int Vn, const unsigned char* var_on) {
-  int rc = maybe_reset_tags(c);
+  int rc = ensure_labels(c);
+  if (rc) return rc;
+  rc = maybe_reset_tags(c);
   if (rc) return rc;
   auto t0 = std::chrono::steady_clock::now();
   Graph G = make_graph(c);
@@ -1424,15 +1449,11 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
   ALLOC(rv_eid, E); ALLOC(tail, E); ALLOC(rvpos, E);
   ALLOC(t0, VE); ALLOC(cap, VE); ALLOC(B, VE); ALLOC(pw, VE); ALLOC(present, VE);
   ALLOC(x, VE); ALLOC(y, VE); ALLOC(c, VE); ALLOC(rvw, VE); ALLOC(fww, VE);
-  const size_t NP = (size_t)N * c->P;
-  ALLOC(dist, NP); ALLOC(pred, NP); ALLOC(pend, NP);
   ALLOC(col_src, c->P); ALLOC(grp_var, c->Galloc);
-  ALLOC(stamp, (size_t)N * c->Galloc);
   c->flags_cap = (int)std::min<int64_t>(std::max<int64_t>(4 * N + 64, 1024), 1 << 24);
   ALLOC(flags, c->flags_cap);
   ALLOC(counters, 8);
-  ALLOC(odc, (size_t)std::max<int64_t>(Z, 1) * c->Pv);
-  ALLOC(ypart, (size_t)c->Galloc * E);
+
   ALLOC(err_dev, 4);
   ALLOC(var_on, V); ALLOC(st, V);
   ALLOC(part, (size_t)V * kLinkBlocks * kQ);
@@ -1451,7 +1472,7 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
   if ((rc = upload(c, c->present, hpres.data(), VE))) return rc;
   if ((rc = upload(c, c->col_src, col_src.data(), c->P))) return rc;
   if ((rc = upload(c, c->grp_var, grp_var.data(), c->Galloc))) return rc;
-  if ((rc = upload(c, c->odc, odc.data(), odc.size()))) return rc;
+  c->h_odc = std::move(odc);
   if ((rc = dalloc(c, &c->od_start, Z + 1))) return rc;
   if ((rc = dalloc(c, &c->od_dest, h_od_dest.size()))) return rc;
   if ((rc = dalloc(c, &c->od_val, h_od_val.size()))) return rc;
@@ -1461,8 +1482,7 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
   if (c->lds_ok)
     HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_origin_lds),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_bytes));
-  HIPCHK(c, hipMemsetAsync(c->stamp, 0xFF, sizeof(int) * (size_t)N * c->Galloc, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->ypart, 0, sizeof(double) * (size_t)c->Galloc * E, c->stream));
+
   HIPCHK(c, hipMemsetAsync(c->x, 0, sizeof(double) * VE, c->stream));
   HIPCHK(c, hipMemsetAsync(c->y, 0, sizeof(double) * VE, c->stream));
   HIPCHK(c, hipMemsetAsync(c->var_on, 1, V, c->stream));
