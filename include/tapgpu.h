@@ -7,8 +7,8 @@
  * Data model (the reference's, trunk/tap/tap_types.h:20-54): node ids are
  * 1-based, zones are nodes 1..num_zones, nodes below first_thru_node are
  * zones that are labelled but never expanded (trunk/tap/sssp_pape.c:355),
- * links are sorted by (init, term) so that link index == CSR edge index
- * (trunk/tap/tap_functions.c:313-339).  OD demand is an origin-major CSR:
+ * links are grouped by init node (the reference sorts them by (init, term))
+ * so that link index == CSR edge index (trunk/tap/tap_functions.c:313-339).  OD demand is an origin-major CSR:
  * origin o's items are [od_start[o-1], od_start[o]); items with dest == o
  * or demand <= 0 are ignored (trunk/tap/tap_frankwolfe.c:146-147).
  *
@@ -40,7 +40,7 @@ typedef struct tapgpu_ctx tapgpu_ctx;
 /* network (one topology); per-link arrays have num_links entries */
 typedef struct {
   int64_t num_zones, num_nodes, first_thru_node, num_links;
-  const int64_t *init_node, *term_node;        /* 1-based, sorted (init, term) */
+  const int64_t *init_node, *term_node;        /* 1-based, grouped by init    */
   const double *capacity, *free_flow_time, *B, *power;
 } tapgpu_network;
 
@@ -123,6 +123,16 @@ int tapgpu_sssp(tapgpu_ctx *ctx, int64_t variant, const int64_t *start_nodes,
    the origin/dest in the error message) where the reference exits. */
 int tapgpu_aon(tapgpu_ctx *ctx, int64_t variant, double *volumes);
 
+/* All-or-nothing loading along caller-supplied shortest-path trees: predlink
+   holds the last link of the path from each zone origin, in the node-major
+   layout predlink[node * ld + col_base + (origin - 1)] (node 1..num_nodes,
+   -1 = none), the layout pape_cuda returns.  Replaces link_volume_update
+   (trunk/tap/volume_update_host.h:18-20, kernel
+   trunk/tap/volume_update_kernels.cu:54-86).  The trees must be the
+   shortest-path forests of the current costs (acyclic). */
+int tapgpu_load_pred(tapgpu_ctx *ctx, int64_t variant, const int64_t *predlink, int64_t ld,
+                     int64_t col_base, double *volumes);
+
 /* --- Frank-Wolfe ----------------------------------------------------------- */
 
 /* Initial solution for every variant: all-or-nothing at free-flow costs
@@ -145,6 +155,9 @@ int tapgpu_get_flows(tapgpu_ctx *ctx, int64_t variant, double *volumes, double *
 /* Copy up to cap per-iteration records of variant v; *count = number held. */
 int tapgpu_get_trace(tapgpu_ctx *ctx, int64_t variant, tapgpu_iter *trace, int cap,
                      int *count);
+
+/* Copy the record of iteration `index` (0-based) of variant v. */
+int tapgpu_get_iter(tapgpu_ctx *ctx, int64_t variant, int index, tapgpu_iter *rec);
 
 /* Device buffers of the current state (for zero-copy consumers such as the
    torch extension): volumes / costs are [num_variants * num_links] FP64 in

@@ -51,3 +51,18 @@ def test_engine_library_exports_every_declared_symbol():
     assert "tapgpu_fw_run" in names
     for n in names:
         assert hasattr(L, n), n
+
+
+def test_host_library_exports_the_reference_api():
+    lib = os.path.join(ROOT, "traffic_assignment_amd", "libtap_hip.so")
+    if not os.path.exists(lib):
+        pytest.skip("libtap_hip.so not built")
+    L = ctypes.CDLL(lib)
+    names = _declared("tap_hip.h")
+    for n in ("tap_frankwolfe", "fw_assign", "pape_cuda", "link_volume_update",
+              "parse_net_file", "parse_trips_file", "print_flow"):
+        assert n in names
+    for n in names:
+        assert hasattr(L, n), n
+    for g in ("num_threads", "num_cores", "using_fermi_architecture"):
+        assert ctypes.c_int.in_dll(L, g) is not None

@@ -1,0 +1,496 @@
+/*
+ * tap_host.c -- the reference's solver entry points (include/tap_hip.h) on
+ * the MI355X engine (include/tapgpu.h).
+ *
+ *   tap_frankwolfe()     trunk/tap/tap_frankwolfe_pthread.c:488-825
+ *   fw_assign()          trunk/tap/tap_frankwolfe_cuda.cu:176-275
+ *   pape_cuda()          trunk/tap/pape_cuda_host.cu:51-195
+ *   link_volume_*()      trunk/tap/volume_update_host.cu:55-247
+ *
+ * The engine context (graph, link records and demand resident in HBM) is
+ * created once and cached; the shims that receive raw arrays every call
+ * (fw_assign, pape_cuda) reuse it while the topology and demand are
+ * unchanged and only upload the new link costs.
+ */
+#include <errno.h>
+#include <float.h>
+#include <math.h>
+#include <signal.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+#include <time.h>
+#include <unistd.h>
+
+#include "tap_hip.h"
+#include "tapgpu.h"
+
+unsigned int num_cores = 1;
+unsigned int num_threads = 1;
+int using_fermi_architecture = 0;
+
+static void die(const char *what, tapgpu_ctx *ctx, int rc)
+{
+  fprintf(stderr, "ERROR: %s failed (%d): %s\n", what, rc, tapgpu_last_error(ctx));
+  exit(1);
+}
+
+/* multiply-xorshift hash over 64-bit words (every input array here is made
+   of 8-byte elements): identifies the topology / demand a cached context was
+   built from */
+static uint64_t fnv(uint64_t h, const void *p, size_t n)
+{
+  const uint64_t *w = p;
+  for (size_t i = 0; i < n / 8; i++) {
+    h ^= w[i] + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 31;
+  }
+  return h;
+}
+
+/* --- network + demand -> engine arrays --------------------------------------- */
+
+typedef struct {
+  int64_t *init, *term, *od_start, *od_dest;
+  double *cap, *t0, *B, *pw, *od_val;
+  tapgpu_network net;
+  tapgpu_demand dem;
+} engine_input;
+
+static void free_input(engine_input *in)
+{
+  free(in->init); free(in->term); free(in->od_start); free(in->od_dest);
+  free(in->cap); free(in->t0); free(in->B); free(in->pw); free(in->od_val);
+  memset(in, 0, sizeof *in);
+}
+
+static int make_input(const net_data_t *net, demand_data_t *const *demands, engine_input *in)
+{
+  memset(in, 0, sizeof *in);
+  const long E = net->num_links, Z = net->num_zones;
+  in->init = malloc(sizeof(int64_t) * (E + 1));
+  in->term = malloc(sizeof(int64_t) * (E + 1));
+  in->cap = malloc(sizeof(double) * (E + 1));
+  in->t0 = malloc(sizeof(double) * (E + 1));
+  in->B = malloc(sizeof(double) * (E + 1));
+  in->pw = malloc(sizeof(double) * (E + 1));
+  in->od_start = calloc((size_t)Z + 1, sizeof(int64_t));
+  if (!in->init || !in->term || !in->cap || !in->t0 || !in->B || !in->pw || !in->od_start)
+    return -1;
+  for (long k = 0; k < E; k++) {
+    in->init[k] = net->links[k].init_node;
+    in->term[k] = net->links[k].term_node;
+    in->cap[k] = net->links[k].capacity;
+    in->t0[k] = net->links[k].free_flow_time;
+    in->B[k] = net->links[k].B;
+    in->pw[k] = net->links[k].power;
+  }
+  long nnz = 0;
+  for (long o = 1; o <= Z && demands; o++)
+    for (long i = 0; demands[o] && demands[o][i].dest != 0; i++)
+      nnz++;
+  in->od_dest = malloc(sizeof(int64_t) * (nnz + 1));
+  in->od_val = malloc(sizeof(double) * (nnz + 1));
+  if (!in->od_dest || !in->od_val)
+    return -1;
+  nnz = 0;
+  for (long o = 1; o <= Z; o++) {
+    for (long i = 0; demands && demands[o] && demands[o][i].dest != 0; i++) {
+      in->od_dest[nnz] = demands[o][i].dest;
+      in->od_val[nnz] = demands[o][i].demand;
+      nnz++;
+    }
+    in->od_start[o] = nnz;
+  }
+  in->net = (tapgpu_network){Z, net->num_nodes, net->first_thru_node, E, in->init, in->term,
+                             in->cap, in->t0, in->B, in->pw};
+  in->dem = (tapgpu_demand){Z, in->od_start, in->od_dest, in->od_val};
+  return 0;
+}
+
+static uint64_t input_key(const engine_input *in)
+{
+  uint64_t h = 1469598103934665603ull;
+  h = fnv(h, &in->net.num_nodes, sizeof(int64_t));
+  h = fnv(h, &in->net.num_zones, sizeof(int64_t));
+  h = fnv(h, &in->net.first_thru_node, sizeof(int64_t));
+  h = fnv(h, in->init, sizeof(int64_t) * in->net.num_links);
+  h = fnv(h, in->term, sizeof(int64_t) * in->net.num_links);
+  h = fnv(h, in->cap, sizeof(double) * in->net.num_links);
+  h = fnv(h, in->t0, sizeof(double) * in->net.num_links);
+  h = fnv(h, in->B, sizeof(double) * in->net.num_links);
+  h = fnv(h, in->pw, sizeof(double) * in->net.num_links);
+  h = fnv(h, in->od_start, sizeof(int64_t) * (in->dem.num_zones + 1));
+  h = fnv(h, in->od_dest, sizeof(int64_t) * in->od_start[in->dem.num_zones]);
+  h = fnv(h, in->od_val, sizeof(double) * in->od_start[in->dem.num_zones]);
+  return h;
+}
+
+/* one cached context per shim family */
+typedef struct {
+  tapgpu_ctx *ctx;
+  uint64_t key;
+} cached_ctx;
+
+static int cached_get(cached_ctx *c, const engine_input *in, tapgpu_ctx **out)
+{
+  const uint64_t key = input_key(in);
+  if (c->ctx && c->key == key) {
+    *out = c->ctx;
+    return TAPGPU_OK;
+  }
+  if (c->ctx) {
+    tapgpu_destroy(c->ctx);
+    c->ctx = NULL;
+  }
+  int rc = tapgpu_create(&c->ctx, &in->net, &in->dem);
+  if (rc)
+    return rc;
+  c->key = key;
+  *out = c->ctx;
+  return TAPGPU_OK;
+}
+
+/* --- fw_assign ------------------------------------------------------------------ */
+
+static cached_ctx g_assign;
+
+/* All-or-nothing assignment of demands at costs Wa into link_volumes
+   (zeroed first).  Va/Ea must be the CSR of net->links (link index ==
+   edge index); dist/predlink are work arrays of the reference and are not
+   written. */
+void fw_assign(net_data_t *net, demand_data_t *demands[], long Va[], long Ea[], double Wa[],
+               double link_volumes[], double dist[], long predlink[])
+{
+  (void)Va; (void)Ea; (void)dist; (void)predlink;
+  engine_input in;
+  if (make_input(net, demands, &in)) {
+    fprintf(stderr, "malloc failed in fw_assign\n");
+    exit(1);
+  }
+  tapgpu_ctx *ctx = NULL;
+  int rc = cached_get(&g_assign, &in, &ctx);
+  if (rc) die("tapgpu_create", NULL, rc);
+  if ((rc = tapgpu_set_costs(ctx, 0, Wa))) die("tapgpu_set_costs", ctx, rc);
+  rc = tapgpu_aon(ctx, 0, link_volumes);
+  if (rc == TAPGPU_E_NOPATH) {
+    fprintf(stderr, "ERROR: %s\n", tapgpu_last_error(ctx));
+    exit(1);
+  }
+  if (rc) die("tapgpu_aon", ctx, rc);
+  free_input(&in);
+}
+
+/* --- pape_cuda ---------------------------------------------------------------- */
+
+static cached_ctx g_pape;
+
+/* All-origin shortest paths over the CSR (Va, Ea, Wa) from start_nodes; the
+   reference's output layout: entry [v * num_start_nodes + s] for start node
+   s and node v (0 <= v < num_nodes). */
+void pape_cuda(long Va[], long Ea[], double Wa[], long num_nodes, long num_edges,
+               long start_nodes[], long num_start_nodes, long first_thru_node,
+               double *distances, long *predecessors)
+{
+  engine_input in;
+  memset(&in, 0, sizeof in);
+  const long N = num_nodes - 1;                  /* node ids 1..N, row 0 unused */
+  in.init = malloc(sizeof(int64_t) * (num_edges + 1));
+  in.term = malloc(sizeof(int64_t) * (num_edges + 1));
+  in.cap = malloc(sizeof(double) * (num_edges + 1));
+  in.t0 = malloc(sizeof(double) * (num_edges + 1));
+  in.B = calloc((size_t)num_edges + 1, sizeof(double));
+  in.pw = malloc(sizeof(double) * (num_edges + 1));
+  in.od_start = calloc(1, sizeof(int64_t));
+  in.od_dest = malloc(sizeof(int64_t));
+  in.od_val = malloc(sizeof(double));
+  if (!in.init || !in.term || !in.cap || !in.t0 || !in.B || !in.pw || !in.od_start) {
+    fprintf(stderr, "malloc failed in pape_cuda\n");
+    exit(1);
+  }
+  for (long v = 0; v < num_nodes; v++)
+    for (long e = Va[v]; e < Va[v + 1]; e++) {
+      in.init[e] = v;
+      in.term[e] = Ea[e];
+    }
+  for (long e = 0; e < num_edges; e++) {
+    in.cap[e] = 1.0;
+    in.t0[e] = 0.0;                              /* topology only in the key */
+    in.pw[e] = 1.0;
+  }
+  if (num_edges > 0 && Va[0] != Va[1]) {
+    fprintf(stderr, "ERROR: pape_cuda: node 0 has links (node ids are 1-based)\n");
+    exit(1);
+  }
+  in.net = (tapgpu_network){0, N, first_thru_node, num_edges, in.init, in.term, in.cap,
+                            in.t0, in.B, in.pw};
+  in.dem = (tapgpu_demand){0, in.od_start, in.od_dest, in.od_val};
+  tapgpu_ctx *ctx = NULL;
+  int rc = cached_get(&g_pape, &in, &ctx);
+  if (rc) die("tapgpu_create", NULL, rc);
+  if ((rc = tapgpu_set_costs(ctx, 0, Wa))) die("tapgpu_set_costs", ctx, rc);
+  int64_t *starts = malloc(sizeof(int64_t) * (num_start_nodes + 1));
+  double *d = malloc(sizeof(double) * (size_t)num_nodes * (num_start_nodes + 1));
+  int64_t *p = malloc(sizeof(int64_t) * (size_t)num_nodes * (num_start_nodes + 1));
+  if (!starts || !d || !p) {
+    fprintf(stderr, "malloc failed in pape_cuda\n");
+    exit(1);
+  }
+  for (long i = 0; i < num_start_nodes; i++)
+    starts[i] = start_nodes[i];
+  rc = tapgpu_sssp(ctx, 0, starts, num_start_nodes, d, p, num_start_nodes, 0);
+  if (rc) die("tapgpu_sssp", ctx, rc);
+  /* column = start node id, as the reference kernel indexes it */
+  for (long v = 0; v < num_nodes; v++)
+    for (long i = 0; i < num_start_nodes; i++) {
+      const size_t o = (size_t)v * num_start_nodes + start_nodes[i];
+      distances[o] = d[(size_t)v * num_start_nodes + i];
+      predecessors[o] = (long)p[(size_t)v * num_start_nodes + i];
+    }
+  free(starts); free(d); free(p);
+  free_input(&in);
+}
+
+/* --- link_volume_* ---------------------------------------------------------------- */
+
+static tapgpu_ctx *g_volume_ctx;
+static long g_volume_links;
+
+void link_volume_data_setup(link_data_t *links, long num_links, long num_start_nodes,
+                            demand_data_t *demands[])
+{
+  long maxnode = 0;
+  for (long k = 0; k < num_links; k++) {
+    if (links[k].init_node > maxnode) maxnode = links[k].init_node;
+    if (links[k].term_node > maxnode) maxnode = links[k].term_node;
+  }
+  net_data_t net = {num_start_nodes, maxnode, 1, num_links, links};
+  engine_input in;
+  if (make_input(&net, demands, &in)) {
+    fprintf(stderr, "malloc failed in link_volume_data_setup\n");
+    exit(1);
+  }
+  link_volume_data_cleanup();
+  int rc = tapgpu_create(&g_volume_ctx, &in.net, &in.dem);
+  if (rc) die("tapgpu_create", NULL, rc);
+  g_volume_links = num_links;
+  free_input(&in);
+}
+
+/* link_volumes := all-or-nothing volumes along the predecessor links
+   predlink[v * num_start_nodes + s] of origins s = 1..num_start_nodes */
+void link_volume_update(long num_start_nodes, long num_edges, long num_nodes,
+                        double link_volumes[], long predlink[])
+{
+  (void)num_nodes;
+  if (!g_volume_ctx || num_edges != g_volume_links) {
+    fprintf(stderr, "ERROR: link_volume_update without link_volume_data_setup\n");
+    exit(1);
+  }
+  int rc = tapgpu_load_pred(g_volume_ctx, 0, (const int64_t *)predlink, num_start_nodes, 1,
+                            link_volumes);
+  if (rc) die("tapgpu_load_pred", g_volume_ctx, rc);
+}
+
+void link_volume_data_cleanup(void)
+{
+  if (g_volume_ctx)
+    tapgpu_destroy(g_volume_ctx);
+  g_volume_ctx = NULL;
+}
+
+/* --- tap_frankwolfe --------------------------------------------------------------- */
+
+static volatile sig_atomic_t g_signal;   /* 0 none, 1 status, 2 checkpoint, 3 stop */
+
+static void on_signal(int sig)
+{
+  if (sig == SIGUSR1) g_signal = 1;
+  else if (sig == SIGUSR2) g_signal = 2;
+  else g_signal = 3;
+}
+
+static int linkflow_compar(const void *a, const void *b)
+{
+  const link_flow_t *x = a, *y = b;
+  if (x->init_node != y->init_node) return x->init_node < y->init_node ? -1 : 1;
+  if (x->term_node != y->term_node) return x->term_node < y->term_node ? -1 : 1;
+  return 0;
+}
+
+static void write_flows(FILE *fp, const net_data_t *net, tapgpu_ctx *ctx, int iter,
+                        double obj, double relgap, double *x, double *c, const char *first)
+{
+  tapgpu_get_flows(ctx, 0, x, c);
+  double vht = 0;
+  for (long k = 0; k < net->num_links; k++)
+    vht += x[k] * c[k];
+  rewind(fp);
+  if (first)
+    fputs(first, fp);
+  fprintf(fp, "~ iterations = %d objvalue = %.15f relgap = %.15f\n", iter, obj, relgap);
+  fprintf(fp, "~ totalcost = %.15f\n", vht);
+  print_flow(fp, net, x, c);
+  fflush(fp);
+  /* a rewritten table may be shorter than a checkpoint written before it;
+     unlike the reference, cut what is left of the older one (regular files) */
+  const long end = ftell(fp);
+  if (end >= 0 && ftruncate(fileno(fp), end) != 0)
+    errno = 0;
+}
+
+/* Frank-Wolfe to target_iterations / target_relgap, writing the flow table
+   to flow_fp; frees demands as the reference does.  Returns 0 or 1. */
+int tap_frankwolfe(net_data_t *net, demand_data_t **demands, int warm_start_mode,
+                   FILE *flows_input_fp, int target_iterations, double target_relgap,
+                   FILE *flow_fp, double *total_cost_at_end, double *objvalue_at_end,
+                   int *iterations_at_end, double *relgap_at_end, int time_each_iteration)
+{
+  const long E = net->num_links;
+  double *warm = NULL;
+  if (warm_start_mode) {
+    link_flow_t *lf = NULL;
+    long nf = 0;
+    if (!flows_input_fp || parse_flows_file(flows_input_fp, &lf, &nf) != 0) {
+      fprintf(stderr, "error parsing flows input file\n");
+      return 1;
+    }
+    fclose(flows_input_fp);
+    qsort(lf, nf, sizeof(link_flow_t), linkflow_compar);
+    if (nf > E) {
+      fprintf(stderr, "error: %ld links in net file but %ld in flows input file\n", E, nf);
+      free(lf);
+      return 1;
+    }
+    /* links added since the flows were written start at zero volume */
+    warm = calloc((size_t)E + 1, sizeof(double));
+    long j = 0;
+    for (long k = 0; k < E; k++)
+      if (j < nf && lf[j].init_node == net->links[k].init_node &&
+          lf[j].term_node == net->links[k].term_node)
+        warm[k] = lf[j++].volume;
+    if (j != nf) {
+      fprintf(stderr, "error: flows input file does not match the network links\n");
+      free(lf); free(warm);
+      return 1;
+    }
+    free(lf);
+  }
+
+  engine_input in;
+  if (make_input(net, demands, &in)) {
+    fprintf(stderr, "malloc failed in tap_frankwolfe\n");
+    free(warm);
+    return 1;
+  }
+  tapgpu_ctx *ctx = NULL;
+  int rc = tapgpu_create(&ctx, &in.net, &in.dem);
+  if (rc) {
+    fprintf(stderr, "ERROR: %s\n", tapgpu_last_error(NULL));
+    free_input(&in); free(warm);
+    return 1;
+  }
+  double *x = malloc(sizeof(double) * (E + 1)), *c = malloc(sizeof(double) * (E + 1));
+  struct sigaction sa, old[7];
+  const int sigs[7] = {SIGUSR1, SIGUSR2, SIGHUP, SIGINT, SIGTERM, SIGXCPU, SIGPWR};
+  memset(&sa, 0, sizeof sa);
+  sa.sa_handler = on_signal;
+  sigemptyset(&sa.sa_mask);
+  g_signal = 0;
+  for (int i = 0; i < 7; i++)
+    sigaction(sigs[i], &sa, &old[i]);
+
+  int iter = 0, ret = 0;
+  double obj = 0, relgap = DBL_MAX;
+  rc = tapgpu_fw_init(ctx, warm);
+  if (rc) {
+    fprintf(stderr, "ERROR: %s\n", tapgpu_last_error(ctx));
+    ret = 1;
+    goto done;
+  }
+  if (warm) {
+    /* the file's costs should be BPR(volume) (reference warns otherwise) */
+    tapgpu_get_flows(ctx, 0, x, c);
+  }
+  /* the device applies the loop test of tap_frankwolfe_pthread.c:712-714
+     every iteration; the host looks at signals between chunks of iterations
+     (every iteration when the per-iteration report is on) */
+  const int chunk = time_each_iteration ? 1 : 16;
+  for (;;) {
+    if (!((target_iterations <= 0 || iter < target_iterations) &&
+          (target_relgap <= 0 || relgap > target_relgap)))
+      break;
+    struct timeval t0, t1;
+    gettimeofday(&t0, NULL);
+    tapgpu_result r;
+    int upto = iter + chunk;
+    if (target_iterations > 0 && upto > target_iterations)
+      upto = target_iterations;
+    rc = tapgpu_fw_run(ctx, upto, target_relgap, &r);
+    if (rc) {
+      fprintf(stderr, "ERROR: %s\n", tapgpu_last_error(ctx));
+      ret = 1;
+      break;
+    }
+    gettimeofday(&t1, NULL);
+    const int ms = (int)((t1.tv_sec - t0.tv_sec) * 1000 + (t1.tv_usec - t0.tv_usec) / 1000);
+    tapgpu_iter t;
+    memset(&t, 0, sizeof t);
+    for (int i = iter; i < r.iterations; i++) {
+      if (tapgpu_get_iter(ctx, 0, i, &t))
+        break;
+      if (time_each_iteration)
+        fprintf(stderr, "iter = %d objective value = %.15f relative gap = %.15f average "
+                "exccess cost = %.15f total cost = %.15f (%d ms)\n", i + 1, t.objective,
+                t.relgap, t.avg_excess, t.vht, ms);
+    }
+    const int progressed = r.iterations > iter;
+    iter = r.iterations;
+    if (progressed) {
+      obj = t.objective;
+      relgap = t.relgap;
+    }
+    if (!time_each_iteration && (g_signal == 1 || g_signal == 2))
+      fprintf(stderr, "iter = %d objective value = %.15f relative gap = %.15f average exccess "
+              "cost = %.15f total cost = %.15f\n", iter, obj, relgap, t.avg_excess, t.vht);
+    if (!progressed)
+      break;
+    if (g_signal == 2) {
+      char buf[128];
+      time_t now = time(NULL);
+      snprintf(buf, sizeof buf, "~ checkpoint written at %s\n", ctime(&now));
+      fprintf(stderr, "checkpointing to flows output file...");
+      write_flows(flow_fp, net, ctx, iter, obj, relgap, x, c, buf);
+      fprintf(stderr, "done\n");
+    }
+    const int stop = g_signal == 3;
+    g_signal = 0;
+    if (stop)
+      break;
+  }
+  if (!ret) {
+    write_flows(flow_fp, net, ctx, iter, obj, relgap, x, c, NULL);
+    double vht = 0;
+    for (long k = 0; k < E; k++)
+      vht += x[k] * c[k];
+    *total_cost_at_end = vht;
+    *objvalue_at_end = obj;
+    *iterations_at_end = iter;
+    *relgap_at_end = relgap;
+  }
+done:
+  for (int i = 0; i < 7; i++)
+    sigaction(sigs[i], &old[i], NULL);
+  tapgpu_destroy(ctx);
+  free_input(&in);
+  free(warm); free(x); free(c);
+  if (demands) {
+    for (long o = 0; o < net->num_zones + 1; o++)
+      free(demands[o]);
+    free(demands);
+  }
+  return ret;
+}

@@ -1340,9 +1340,8 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
     if (a < 1 || a > N || b < 1 || b > N)
       return c->fail(TAPGPU_E_ARG, "link %lld has node out of range (%lld, %lld)", (long long)k,
                      (long long)a, (long long)b);
-    if (k > 0 && (a < net->init_node[k - 1] ||
-                  (a == net->init_node[k - 1] && b < net->term_node[k - 1])))
-      return c->fail(TAPGPU_E_ARG, "links not sorted by (init, term) at %lld", (long long)k);
+    if (k > 0 && a < net->init_node[k - 1])
+      return c->fail(TAPGPU_E_ARG, "links not grouped by init node at %lld", (long long)k);
     fw_off[a]++;  // count into slot a (0-based a-1 -> prefix below)
     rv_off[b]++;
     fw_dst[k] = (int)(b - 1);
@@ -1839,6 +1838,51 @@ int tapgpu_get_trace(tapgpu_ctx* c, int64_t variant, tapgpu_iter* trace, int cap
   if (trace && n > 0)
     HIPCHK(c, hipMemcpy(trace, c->trace + (size_t)variant * c->trace_cap, sizeof(tapgpu_iter) * n,
                         hipMemcpyDeviceToHost));
+  return TAPGPU_OK;
+}
+
+int tapgpu_load_pred(tapgpu_ctx* c, int64_t variant, const int64_t* predlink, int64_t ld,
+                     int64_t col_base, double* volumes) {
+  if (!c || !predlink) return TAPGPU_E_ARG;
+  int rc = check_variant(c, variant);
+  if (rc) return rc;
+  if ((rc = ensure_labels(c))) return rc;
+  const int v = (int)variant;
+  // predecessor links of zone origins into the variant's columns
+  std::vector<int> hp((size_t)c->N * c->Pv, -1);
+  for (int64_t node = 1; node <= c->N; ++node)
+    for (int64_t j = 0; j < c->Z; ++j) {
+      const int64_t e = predlink[(size_t)node * ld + col_base + c->zone_order[j]];
+      if (e < -1 || e >= c->E)
+        return c->fail(TAPGPU_E_ARG, "predecessor link %lld out of range", (long long)e);
+      hp[(size_t)(node - 1) * c->Pv + j] = (int)e;
+    }
+  HIPCHK(c, hipMemcpy2DAsync(c->pred + (size_t)v * c->Pv, c->P * sizeof(int), hp.data(),
+                             c->Pv * sizeof(int), c->Pv * sizeof(int), c->N,
+                             hipMemcpyHostToDevice, c->stream));
+  std::vector<unsigned char> on(c->V, 0);
+  on[v] = 1;
+  unsigned char* d_on = nullptr;
+  if ((rc = dalloc(c, &d_on, c->V))) return rc;
+  rc = upload(c, d_on, on.data(), c->V);
+  if (!rc) rc = aon_groups(c, v * c->Gv, c->Gv, v, 1, d_on);
+  if (!rc && volumes) {
+    const hipError_t e = hipMemcpy(volumes, c->y + (size_t)v * c->E, sizeof(double) * c->E,
+                                   hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = c->fail(TAPGPU_E_HIP, "copy of volumes failed");
+  }
+  c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void*)d_on), c->allocs.end());
+  (void)hipFree(d_on);
+  return rc;
+}
+
+int tapgpu_get_iter(tapgpu_ctx* c, int64_t variant, int index, tapgpu_iter* rec) {
+  if (!c || !rec) return TAPGPU_E_ARG;
+  int rc = check_variant(c, variant);
+  if (rc) return rc;
+  if (index < 0 || index >= c->trace_cap) return c->fail(TAPGPU_E_ARG, "iteration %d not held", index);
+  HIPCHK(c, hipMemcpy(rec, c->trace + (size_t)variant * c->trace_cap + index, sizeof(tapgpu_iter),
+                      hipMemcpyDeviceToHost));
   return TAPGPU_OK;
 }
 
