@@ -199,6 +199,8 @@ def main():
                     help="batched: variants solved one by one for the plain comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--strategy", default="auto", choices=["auto", "sweeps", "lds"])
+    ap.add_argument("--max-iters", type=int, default=0,
+                    help="cap FW iterations per step (profiling runs; 0 = solve to 1e-4)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
@@ -219,7 +221,7 @@ def main():
         eng.set_strategy(args.strategy)
     for _ in range(args.warmup):
         eng.fw_init()
-        eng.fw_run(target_relgap=TARGET)
+        eng.fw_run(max_iters=args.max_iters, target_relgap=TARGET)
     eng.set_profiling(True)
     acc = {}
     torch.cuda.synchronize()
@@ -227,7 +229,7 @@ def main():
     iters = []
     for _ in range(args.steps):
         eng.fw_init()
-        r = eng.fw_run(target_relgap=TARGET)[0]
+        r = eng.fw_run(max_iters=args.max_iters, target_relgap=TARGET)[0]
         iters.append(r["iterations"])
         for k, v in eng.stats().items():
             acc[k] = acc.get(k, 0) + v

@@ -212,3 +212,25 @@ def test_zero_demand_and_isolated(engine_mod):
         x, c = eng.flows()
     assert np.all(x == 0) and np.array_equal(c, net.fftime)
     assert r["iterations"] == 3
+
+
+def test_python_pape_module(engine_mod, oracle_lib):
+    """The reference's Python operator interface: predecessor nodes."""
+    from traffic_assignment_amd.python_extension import pape
+    net, _ = network("metro", 0.3)
+    graph = {}
+    for u, v, c in zip(net.init, net.term, net.fftime):
+        graph.setdefault(int(u), {})[int(v)] = float(c)
+    pape.build_adjlist(graph, net.num_nodes + 1, net.first_thru_node)
+    rng = np.random.default_rng(4)
+    w = net.fftime * rng.uniform(1, 2, net.num_links)
+    pape.update_weights(list(w))
+    for s in (1, 17, net.num_zones):
+        preds = pape.pape(s)
+        _, op = oracle_lib.sssp(net, w, s)
+        ref = [-1 if k < 0 else int(net.init[k]) for k in op]
+        assert preds == ref
+    with pytest.raises(IndexError):
+        pape.update_weights([1.0])
+    with pytest.raises(IndexError):
+        pape.pape(net.num_nodes + 1)
