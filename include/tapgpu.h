@@ -205,6 +205,42 @@ int tapgpu_set_profiling(tapgpu_ctx *ctx, int on);
 int tapgpu_set_strategy(tapgpu_ctx *ctx, int strategy);
 int tapgpu_get_strategy(const tapgpu_ctx *ctx);
 
+/* --- equilibrium gradients ------------------------------------------------ */
+
+typedef struct {
+  int64_t cycles;           /* active cycles (origin, tight non-tree link)   */
+  int64_t nonzeros;         /* links summed over all cycles                   */
+  int32_t useful_rounds;    /* rounds of the usefulness fixpoint              */
+  int32_t cg_iters;         /* conjugate-gradient iterations                  */
+  int32_t cg_status;        /* 1 converged, 2 breakdown, 0 iteration limit    */
+  int32_t reserved;
+  double cg_relres;         /* preconditioned residual / initial residual    */
+  double setup_ms, cg_ms;   /* wall time: shortest paths + cycle matrix, CG  */
+} tapgpu_grad_info;
+
+/* Gradients at the variant's current volumes (after tapgpu_fw_run, or
+   tapgpu_fw_init with warm volumes), which should be a user equilibrium.
+   The reference has no counterpart; the operator boundary it serves is the
+   Python extension (trunk/python_extension/pape_module.c:301-473), whose
+   successor carries a torch.autograd.Function over this call.
+     dobj_dt0, dobj_dcap  d(Beckmann objective)/d(free-flow time, capacity)
+                          by the envelope theorem;
+     dvht_dcap, dvht_dt0  d(sum_a x_a t_a)/d(capacity, free-flow time) by
+                          implicit differentiation over the active paths:
+                          links with reduced cost <= eps * label of their head
+                          in some origin's shortest-path tree, on the way to
+                          a destination with demand; a conjugate-gradient
+                          solve to relative residual cg_tol, at most max_cg
+                          iterations.  Link curvatures dt/dx below 1e-8 of
+                          the largest count as zero: detours of constant cost
+                          keep their (undetermined) flow split.
+   Any output pointer may be NULL; each has num_links entries (0 for links
+   absent from the variant).  Uses the node-major label buffers (allocated on
+   first use) and temporary device memory; the FW state is unchanged. */
+int tapgpu_gradient(tapgpu_ctx *ctx, int64_t variant, double eps, double cg_tol, int max_cg,
+                    double *dobj_dt0, double *dobj_dcap, double *dvht_dcap, double *dvht_dt0,
+                    tapgpu_grad_info *info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -95,6 +95,20 @@ int ora_fw_solve(const ora_net *net, const ora_demand *dem, int max_iters,
                  double *x_out, double *c_out, ora_iter *trace, int trace_cap,
                  int *iters_out, double *relgap_out, double *obj_out, double *vht_out);
 
+/* Path-based equilibrium (equilibrium.c): gradient projection with column
+   generation, for the finite-difference pins of the gradients.  The handle
+   is opaque; see equilibrium.c for the method. */
+void *ora_gp_create(const ora_net *net, const ora_demand *dem);
+void ora_gp_destroy(void *h);
+void ora_gp_set_params(void *h, const double *t0, const double *cap);
+double ora_gp_relgap(void *h);
+int ora_gp_run(void *h, int max_sweeps, double tol, double *relgap);
+void ora_gp_flows(void *h, double *x, double *t);
+void ora_gp_count(void *h, int64_t *num_paths, int64_t *num_links);
+void ora_gp_paths(void *h, int64_t *od, double *flow, int64_t *off, int64_t *links,
+                  int64_t *od_origin, int64_t *od_dest, double *od_demand);
+int64_t ora_gp_num_od(void *h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1921,3 +1921,5 @@ int tapgpu_set_profiling(tapgpu_ctx* c, int on) {
 }
 
 }  // extern "C"
+
+#include "tapgrad.inc"

@@ -69,6 +69,12 @@ class _Stats(C.Structure):
                 ("origin_loaded", C.c_int64)]
 
 
+class _GradInfo(C.Structure):
+    _fields_ = [("cycles", C.c_int64), ("nonzeros", C.c_int64), ("useful_rounds", C.c_int32),
+                ("cg_iters", C.c_int32), ("cg_status", C.c_int32), ("reserved", C.c_int32),
+                ("cg_relres", C.c_double), ("setup_ms", C.c_double), ("cg_ms", C.c_double)]
+
+
 _lib = None
 vp = C.c_void_p
 
@@ -100,6 +106,8 @@ def lib():
         L.tapgpu_set_profiling.argtypes = [vp, C.c_int]
         L.tapgpu_set_strategy.argtypes = [vp, C.c_int]
         L.tapgpu_get_strategy.argtypes = [vp]
+        L.tapgpu_gradient.argtypes = [vp, C.c_int64, C.c_double, C.c_double, C.c_int,
+                                      vp, vp, vp, vp, C.POINTER(_GradInfo)]
         _lib = L
     return _lib
 
@@ -229,6 +237,20 @@ class TapEngine:
         x, c = vp(), vp()
         self._check(lib().tapgpu_device_state(self._h, C.byref(x), C.byref(c)))
         return x.value, c.value
+
+    def gradient(self, variant=0, eps=1e-9, cg_tol=1e-12, max_cg=20000):
+        """Equilibrium gradients at the current volumes (tapgpu_gradient):
+        dict of per-link arrays dobj_dt0, dobj_dcap, dvht_dcap, dvht_dt0 and
+        an "info" dict (cycles, nonzeros, cg_iters, cg_relres, ...)."""
+        out = {k: np.empty(self.E, dtype=np.float64)
+               for k in ("dobj_dt0", "dobj_dcap", "dvht_dcap", "dvht_dt0")}
+        info = _GradInfo()
+        self._check(lib().tapgpu_gradient(self._h, variant, float(eps), float(cg_tol), int(max_cg),
+                                          _p(out["dobj_dt0"]), _p(out["dobj_dcap"]),
+                                          _p(out["dvht_dcap"]), _p(out["dvht_dt0"]),
+                                          C.byref(info)))
+        out["info"] = {f: getattr(info, f) for f, _ in _GradInfo._fields_ if f != "reserved"}
+        return out
 
     STRATEGIES = {"auto": 0, "sweeps": 1, "lds": 2}
 
