@@ -7,7 +7,9 @@ costs, then iterations until relgap <= 1e-4) of the workload, with the
 network and demand already resident in HBM (uploaded before timing).
 Workloads: metro (default: jittered 100x100 grid + 1,000 zones, 43,600
 links), grid5 (24 nodes), regional (300x300 grid + 4,000 zones), batched
-(512 upgrade-subset variants of metro; value = variant solves/sec).
+(512 upgrade-subset variants of metro; value = variant solves/sec),
+gradient (equilibrium gradients on metro at relgap 1e-6; value =
+gradients/sec, beside the cost of finite-difference re-solves).
 
 Prints one JSON line with the throughput, the roofline of the dominant
 kernel (the shortest-path sweep) measured with HIP events around every
@@ -56,12 +58,13 @@ def origin_bytes(st, num_links) -> float:
     origin per row (8 B per expansion + its label, 12 B of link data and a
     label per relaxed link, 12 B per lowered label), plus predecessor
     selection (per link: 16 B of reverse-CSR data and the tail's label) and
-    loading (per tree node: 12 B of tree data, 8 B of flow, 16 B link-volume
-    read-modify-write)."""
+    loading (per walk step 2 B of tree data and 8 B of flow
+    read-modify-write; per tree link with flow 8 B of flow, 2 B of tree data
+    and a 16 B link-volume read-modify-write)."""
     F, Es, U = st["origin_expansions"], st["origin_scanned"], st["origin_updates"]
     relax = 8.0 * F + 12.0 * Es + 8.0 * (F + Es) + 12.0 * U
     pred = 24.0 * num_links * st["origin_blocks"]
-    load = 36.0 * st["origin_loaded"]
+    load = 10.0 * st["origin_walk_steps"] + 26.0 * st["origin_loaded"]
     return relax + pred + load
 
 
@@ -187,13 +190,76 @@ def bench_batched(args):
     print(json.dumps(out))
 
 
+def bench_gradient(args):
+    """Gradient form on metro: one cold FW solve to relgap 1e-6, then --steps
+    timed gradient calls (envelope d Beckmann/d(c, t0) and implicit
+    d VHT/d(c, t0) for every link), against warm re-solves after a 1e-4
+    capacity change -- what one central finite difference per link costs."""
+    import torch
+    from traffic_assignment_amd import synth
+    from traffic_assignment_amd.engine import TapEngine
+
+    target = 1e-6
+    net, dem = synth.make("metro")
+    eng = TapEngine(net, dem)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    eng.fw_init()
+    r = eng.fw_run(max_iters=args.max_iters or 100000, target_relgap=target)[0]
+    torch.cuda.synchronize()
+    solve_s = time.perf_counter() - t
+    x, _ = eng.flows()
+    eps = max(1e-9, 20 * r["relgap"])
+    for _ in range(args.warmup):
+        eng.gradient(eps=eps, cg_tol=1e-10)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        g = eng.gradient(eps=eps, cg_tol=1e-10)
+    torch.cuda.synchronize()
+    grad_s = (time.perf_counter() - t) / args.steps
+    eng.close()
+    cap = np.array(net.capacity, dtype=np.float64)
+    k = int(np.argmax(x / cap))
+    warm = []
+    for sgn in (1.0, -1.0):
+        cp = cap.copy()
+        cp[k] *= 1.0 + sgn * 1e-4
+        with TapEngine(net, dem, num_variants=1, capacity=cp[None, :]) as e2:
+            e2.fw_init(x)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            e2.fw_run(max_iters=100000, target_relgap=target)
+            torch.cuda.synchronize()
+            warm.append(time.perf_counter() - t)
+    resolve = float(np.mean(warm))
+    info = g["info"]
+    out = {
+        "metric": "equilibrium gradients/sec (d Beckmann, d VHT w.r.t. capacity and t0, all links)",
+        "value": round(1.0 / grad_s, 4), "unit": "gradients/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * grad_s, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded generator, traffic_assignment_amd/synth.py)",
+        "config": {"workload": "gradient form on " + DESCR["metro"] + " at relgap 1e-6 (the "
+                               "regional network at 1e-6 needs ~1e4+ FW iterations of 0.5 s)",
+                   "fw_iterations": r["iterations"], "relgap": r["relgap"], "eps": eps},
+        "solve_to_1e-6_s": round(solve_s, 3),
+        "warm_resolve_s": round(resolve, 4),
+        "fd_64x2_warm_resolves_s": round(128 * resolve, 2),
+        "fd_64x2_cold_resolves_s": round(128 * solve_s, 1),
+        "gradient_vs_64x2_warm_resolves": round(128 * resolve / grad_s, 1),
+        "gradient_info": info,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="metro",
-                    choices=["grid5", "metro", "regional", "batched"])
+                    choices=["grid5", "metro", "regional", "batched", "gradient"])
     ap.add_argument("--variants", type=int, default=512, help="batched: number of variants")
     ap.add_argument("--plain-sample", type=int, default=8,
                     help="batched: variants solved one by one for the plain comparison")
@@ -215,6 +281,8 @@ def main():
 
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "gradient":
+        return bench_gradient(args)
     net, dem = synth.make(args.workload)
     eng = TapEngine(net, dem)          # graph, link data and demand -> HBM
     if args.strategy != "auto":

@@ -186,8 +186,15 @@ typedef struct {
   int64_t origin_expansions;   /* node expansions (label-correcting)        */
   int64_t origin_scanned;      /* out-links relaxed                         */
   int64_t origin_updates;      /* labels lowered                            */
-  int64_t origin_relax_rounds, origin_load_rounds;  /* summed over origins  */
-  int64_t origin_loaded;       /* tree nodes loaded                         */
+  int64_t origin_relax_rounds;  /* relaxation rounds, summed over origins    */
+  int64_t origin_load_rounds;   /* longest loading walk, summed over origins */
+  int64_t origin_loaded;        /* tree links given flow                     */
+  int64_t origin_walk_steps;    /* destination-to-origin walk steps          */
+  double origin_relax_ms;       /* workgroup time per phase, summed over     */
+  double origin_pred_ms;        /*   workgroups (device wall clock)          */
+  double origin_load_ms;
+  double origin_tune_ms;        /* fastest launch of the schedule tuning     */
+  int64_t origin_chase;         /* chained expansions per thread and round   */
 } tapgpu_stats;
 
 /* Counters and wall times of the last tapgpu_fw_run (or of the calls since
@@ -203,6 +210,14 @@ int tapgpu_set_profiling(tapgpu_ctx *ctx, int on);
    fewer than 65535 links).  tapgpu_get_strategy returns the one in use
    (1 or 2). */
 int tapgpu_set_strategy(tapgpu_ctx *ctx, int strategy);
+
+/* Worklist schedule of the one-origin-per-workgroup path: how many labels a
+   thread lowers and then expands itself, in a chain, within one round
+   (0: every lowered node waits for the next round).  -1 (the default):
+   tuned on the first loading of the context by timing each candidate
+   (0, 2, 4, 8, 16, 32) on the real launch.  Results do not depend on it. */
+int tapgpu_set_origin_chase(tapgpu_ctx *ctx, int chase);
+int tapgpu_get_origin_chase(const tapgpu_ctx *ctx);
 int tapgpu_get_strategy(const tapgpu_ctx *ctx);
 
 /* --- equilibrium gradients ------------------------------------------------ */

@@ -407,9 +407,10 @@ __global__ __launch_bounds__(kBlock) void k_aon_reduce(int E, int Gv, int v0, in
 // 160 KB LDS, so a 1024-thread workgroup runs the whole origin: rounds of
 // label-correcting relaxation (frontier bitmaps, 64-bit LDS atomicMin on the
 // label bits -- non-negative doubles order like their bit patterns), then
-// predecessor selection, then bottom-up loading of the tree with LDS FP64
-// atomics, adding link volumes to HBM with FP64 atomics.  One launch covers
-// every (variant, origin).
+// predecessor selection, then loading: each destination walks its tree path
+// adding its demand to LDS node flows (FP64 LDS atomics), and every tree
+// link with flow adds it to the HBM link volumes (FP64 atomics).  One launch
+// covers every (variant, origin).
 //
 // Predecessor rule: among in-links u->v with fl(dist[u] + w) == dist[v]
 // (u expandable), take the tail with the smallest label (the one a
@@ -420,6 +421,7 @@ __global__ __launch_bounds__(kBlock) void k_aon_reduce(int E, int Gv, int v0, in
 // this is the reference's predecessor.
 
 constexpr int kLdsBlock = 1024;
+constexpr int kDefaultChase = 4;               // before tuning / shortest-path-only calls
 constexpr int kLdsMaxNodes = 16384;            // per-thread register arrays below
 constexpr int kLdsPerThread = kLdsMaxNodes / kLdsBlock;
 constexpr unsigned short kNoPred = 0xFFFF;
@@ -433,10 +435,14 @@ struct OriginArgs {
   const int* od_dest;                // 0-based zone
   const double* od_val;
   const unsigned char* var_on;       // [V] or null
+  int chase;                         // expansions a thread chains per round
   double* y;                         // [V * E], zeroed by the caller
   int* err;                          // [3]
-  unsigned long long* counters;      // [8] expansions, links scanned, updates,
-                                     //     relax rounds, load rounds, loaded nodes
+  unsigned long long* counters;      // [16] expansions, links scanned, updates,
+                                     //     relax rounds, longest loading walk,
+                                     //     tree links loaded, walk steps, -,
+                                     //     wall-clock ticks of relaxation,
+                                     //     predecessors, loading
   // shortest-path-only mode: block x handles start node src_list[x]
   const int* src_list;
   double* out_dist;                  // [(N) * ld] node-major, row v = node v+1
@@ -447,11 +453,12 @@ struct OriginArgs {
 __device__ __forceinline__ double as_d(unsigned long long b) { return __longlong_as_double((long long)b); }
 __device__ __forceinline__ unsigned long long as_b(double d) { return (unsigned long long)__double_as_longlong(d); }
 
-// LDS layout (bytes): labels 8N | three bitmaps 3*8W | offs/pred 2A |
-// rnd/ptail 2A | pend A, with W = ceil(N/64), A = N+1 rounded up to 4.
+// LDS layout (bytes): labels 8N (flows while loading) | three bitmaps
+// 3*8W | offs/pred 2A | rnd/ptail 2A, with W = ceil(N/64), A = N+1 rounded
+// up to 4.
 size_t origin_lds_bytes(int N) {
   const size_t W = (size_t)(N + 63) / 64, A = (size_t)((N + 1 + 3) & ~3);
-  return 8ull * N + 24ull * W + 5ull * A;
+  return 8ull * N + 24ull * W + 4ull * A;
 }
 
 __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load) {
@@ -467,7 +474,6 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
   unsigned short* pred = offs;
   unsigned short* rnd = offs + Al;                                 // [N], then ptail [N]
   unsigned short* ptail = rnd;
-  unsigned* pendw = (unsigned*)(rnd + Al);                         // [A/4] packed bytes
   __shared__ int s_more[3];
   __shared__ unsigned long long s_cnt[8];
 
@@ -492,8 +498,13 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
   __syncthreads();
 
   unsigned long long n_exp = 0, n_scan = 0, n_upd = 0;
+  const long long clk0 = wall_clock64();
   // ---- label-correcting rounds (one barrier each: three rotating bitmaps,
-  //      the third one is cleared while the first is read) ----
+  //      the third one is cleared while the first is read).  A thread that
+  //      lowers a label expands that node itself, up to A.chase times in a
+  //      row, instead of leaving it to the next round: chains of several
+  //      hops settle per barrier.  Labels are the same least fixed point for
+  //      any such order. ----
   unsigned long long *cur = bmA, *nxt = bmB, *clr = bmC;
   int r = 1;
   for (;; ++r) {
@@ -503,26 +514,45 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
       while (bits) {
         const int b = __ffs(bits) - 1;
         bits &= bits - 1;
-        const int u = (w << 6) + 16 * q + b;
-        const double du = as_d(dist[u]);
-        const int eb = offs[u], ee = offs[u + 1];
-        ++n_exp;
-        n_scan += (unsigned long long)(ee - eb);
-        for (int e = eb; e < ee; ++e) {
-          const int v = A.fw_dst[e];
-          const double cand = du + fw[e];
-          const unsigned long long cb = as_b(cand);
-          if (cb < dist[v]) {
-            const unsigned long long old = atomicMin(&dist[v], cb);
-            if (cb < old) {
-              ++n_upd;
-              rnd[v] = (unsigned short)r;
-              if (v >= A.ft0) {
-                atomicOr(&nxt[v >> 6], 1ull << (v & 63));
-                s_more[r % 3] = 1;
+        int u = (w << 6) + 16 * q + b;
+        for (int hops = 0;; ++hops) {
+          const double du = as_d(dist[u]);
+          const int eb = offs[u], ee = offs[u + 1];
+          ++n_exp;
+          n_scan += (unsigned long long)(ee - eb);
+          int next = -1;
+          // up to 8 links' heads and costs are loaded before any of them is
+          // relaxed: one L2 round trip per expansion instead of one per link
+          for (int e0 = eb; e0 < ee; e0 += 8) {
+            int hv[8];
+            double hw[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+              if (e0 + k < ee) { hv[k] = A.fw_dst[e0 + k]; hw[k] = fw[e0 + k]; }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+              if (e0 + k >= ee) break;
+              const int v = hv[k];
+              const unsigned long long cb = as_b(du + hw[k]);
+              if (cb < dist[v]) {
+                const unsigned long long old = atomicMin(&dist[v], cb);
+                if (cb < old) {
+                  ++n_upd;
+                  rnd[v] = (unsigned short)r;
+                  if (v >= A.ft0) {
+                    if (next < 0 && hops < A.chase) {
+                      next = v;
+                    } else {
+                      atomicOr(&nxt[v >> 6], 1ull << (v & 63));
+                      s_more[r % 3] = 1;
+                    }
+                  }
+                }
               }
             }
           }
+          if (next < 0) break;
+          u = next;
         }
       }
     }
@@ -537,6 +567,7 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
     unsigned long long* t = cur; cur = nxt; nxt = clr; clr = t;
   }
   const int relax_rounds = r;
+  const long long clk1 = wall_clock64();
 
   // ---- predecessor links (label, then round/node key; see above) ----
   unsigned short my_tail[kLdsPerThread];
@@ -573,6 +604,7 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
     }
   }
   __syncthreads();        // every rnd[] read is done
+  const long long clk2 = wall_clock64();
   {
     int j = 0;
     for (int v = tid; v < N; v += kLdsBlock, ++j) ptail[v] = my_tail[j];
@@ -589,89 +621,67 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
     return;
   }
 
-  // ---- all-or-nothing loading, bottom-up on the tree ----
+  // ---- all-or-nothing loading: every destination with demand walks its
+  //      tree path to the origin, adding its demand to each node's flow
+  //      (the flow on the node's tree in-link); no rounds, no barriers.
+  //      Integer demands make every partial sum exact, so the order of the
+  //      LDS atomics does not matter. ----
   double* flow = (double*)dist;
   for (int v = tid; v < N; v += kLdsBlock) flow[v] = 0.0;
-  for (int i = tid; i < Al / 4; i += kLdsBlock) pendw[i] = 0u;
-  for (int w = tid; w < 3 * W; w += kLdsBlock) bmA[w] = 0;
-  __shared__ int s_tree, s_done;
-  if (tid < 3) s_more[tid] = 0;
-  if (tid == 0) { s_tree = 0; s_done = 0; }
+  __shared__ unsigned long long s_steps;
+  __shared__ int s_maxlen, s_loaded;
+  if (tid == 0) { s_steps = 0; s_maxlen = 0; s_loaded = 0; }
   __syncthreads();
+  unsigned long long steps = 0;
+  int maxlen = 0;
   for (long long i = A.od_start[src] + tid; i < A.od_start[src + 1]; i += kLdsBlock) {
     const int d = A.od_dest[i];
     const double f = A.od_val[i];
     if (d == src || !(f > 0.0)) continue;
-    atomicAdd(&flow[d], f);
-    if (pred[d] == kNoPred && atomicCAS(&A.err[0], 0, 1) == 0) { A.err[1] = src; A.err[2] = d; }
-  }
-  int in_tree = 0;
-  for (int v = tid; v < N; v += kLdsBlock) {
-    if (pred[v] == kNoPred) continue;
-    ++in_tree;
-    const int p = ptail[v];
-    if (p != src) atomicAdd(&pendw[p >> 2], 1u << ((p & 3) * 8));
-  }
-  atomicAdd(&s_tree, in_tree);
-  __syncthreads();
-  for (int w = tid; w < W; w += kLdsBlock) {
-    unsigned long long bits = 0;
-    for (int b = 0; b < 64; ++b) {
-      const int v = (w << 6) + b;
-      if (v >= N) break;
-      if (pred[v] != kNoPred && ((pendw[v >> 2] >> ((v & 3) * 8)) & 0xFFu) == 0u)
-        bits |= 1ull << b;
+    if (pred[d] == kNoPred) {
+      if (atomicCAS(&A.err[0], 0, 1) == 0) { A.err[1] = src; A.err[2] = d; }
+      continue;
     }
-    bmA[w] = bits;
-  }
-  __syncthreads();
-  cur = bmA; nxt = bmB; clr = bmC;
-  double* __restrict__ yv = A.y + (size_t)var * A.E;
-  int done = 0;
-  for (r = 1;; ++r) {
-    for (int slot = tid; slot < 4 * W; slot += kLdsBlock) {
-      const int w = slot >> 2, q = slot & 3;
-      unsigned bits = (unsigned)((cur[w] >> (16 * q)) & 0xFFFFull);
-      while (bits) {
-        const int b = __ffs(bits) - 1;
-        bits &= bits - 1;
-        const int v = (w << 6) + 16 * q + b;
-        const int p = ptail[v];
-        const double f = flow[v];
-        ++done;
-        if (f != 0.0) {
-          unsafeAtomicAdd(&yv[pred[v]], f);
-          if (p != src) atomicAdd(&flow[p], f);
-        }
-        if (p != src) {
-          const unsigned sh = (p & 3) * 8;
-          const unsigned old = atomicSub(&pendw[p >> 2], 1u << sh);
-          if (((old >> sh) & 0xFFu) == 1u) {
-            atomicOr(&nxt[p >> 6], 1ull << (p & 63));
-            s_more[r % 3] = 1;
-          }
-        }
+    int v = d, len = 0;
+    while (v != src) {
+      if (len > N) {        // a predecessor cycle
+        if (atomicCAS(&A.err[0], 0, 3) == 0) { A.err[1] = src; A.err[2] = 0; }
+        break;
       }
+      atomicAdd(&flow[v], f);
+      v = ptail[v];
+      ++len;
     }
-    for (int w = tid; w < W; w += kLdsBlock) clr[w] = 0;
-    if (tid == 0) s_more[(r + 1) % 3] = 0;
-    __syncthreads();
-    if (!s_more[r % 3]) break;
-    unsigned long long* t = cur; cur = nxt; nxt = clr; clr = t;
+    steps += (unsigned long long)len;
+    maxlen = max(maxlen, len);
   }
-  // every tree node must have been loaded (a predecessor cycle would not be)
-  atomicAdd(&s_done, done);
+  __syncthreads();
+  double* __restrict__ yv = A.y + (size_t)var * A.E;
+  int loaded = 0;
+  for (int v = tid; v < N; v += kLdsBlock) {
+    const double f = flow[v];
+    if (f != 0.0 && pred[v] != kNoPred) {
+      unsafeAtomicAdd(&yv[pred[v]], f);
+      ++loaded;
+    }
+  }
   if (A.counters) {
     atomicAdd(&s_cnt[0], n_exp); atomicAdd(&s_cnt[1], n_scan); atomicAdd(&s_cnt[2], n_upd);
+    atomicAdd(&s_steps, steps);
+    atomicMax(&s_maxlen, maxlen);
+    atomicAdd(&s_loaded, loaded);
   }
   __syncthreads();
-  if (tid == 0) {
-    if (s_done != s_tree && atomicCAS(&A.err[0], 0, 3) == 0) { A.err[1] = src; A.err[2] = 0; }
-    if (A.counters) {
-      atomicAdd(&A.counters[0], s_cnt[0]); atomicAdd(&A.counters[1], s_cnt[1]);
-      atomicAdd(&A.counters[2], s_cnt[2]); atomicAdd(&A.counters[3], (unsigned long long)relax_rounds);
-      atomicAdd(&A.counters[4], (unsigned long long)r); atomicAdd(&A.counters[5], (unsigned long long)s_done);
-    }
+  if (tid == 0 && A.counters) {
+    atomicAdd(&A.counters[0], s_cnt[0]); atomicAdd(&A.counters[1], s_cnt[1]);
+    atomicAdd(&A.counters[2], s_cnt[2]); atomicAdd(&A.counters[3], (unsigned long long)relax_rounds);
+    atomicAdd(&A.counters[4], (unsigned long long)s_maxlen);
+    atomicAdd(&A.counters[5], (unsigned long long)s_loaded);
+    atomicAdd(&A.counters[6], s_steps);
+    const long long clk3 = wall_clock64();
+    atomicAdd(&A.counters[8], (unsigned long long)(clk1 - clk0));
+    atomicAdd(&A.counters[9], (unsigned long long)(clk2 - clk1));
+    atomicAdd(&A.counters[10], (unsigned long long)(clk3 - clk2));
   }
 }
 
@@ -911,6 +921,8 @@ struct tapgpu_ctx {
   double* od_val = nullptr;
   // strategy: 0 auto, 1 node-major sweeps, 2 one origin per workgroup (LDS)
   int strategy = 0;
+  int chase = -1;                     // LDS path: chained expansions (-1: tune)
+  int clock_khz = 0;                  // device wall clock
   bool labels_ready = false;          // node-major buffers allocated
   std::vector<double> h_odc;          // demand by (zone row, column), host copy
   bool lds_ok = false;
@@ -1172,27 +1184,75 @@ OriginArgs make_origin_args(tapgpu_ctx* c, const unsigned char* var_on) {
   A.rv_eid = c->rv_eid; A.tail = c->tail; A.fww = c->fww; A.rvw = c->rvw;
   A.od_start = c->od_start; A.od_dest = c->od_dest; A.od_val = c->od_val;
   A.var_on = var_on; A.y = c->y; A.err = c->err_dev; A.counters = c->counters;
+  A.chase = c->chase >= 0 ? c->chase : kDefaultChase;
   return A;
 }
 
 // Shortest paths + loading for variants [v0, v0+Vn) with one workgroup per
 // (origin, variant); y of those variants is overwritten.
-int aon_lds(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
-  auto t0 = std::chrono::steady_clock::now();
+//
+// Worklist schedule tuning: the first loading on a context with chase < 0
+// times one full launch for each candidate chain length and keeps the
+// fastest (the result is the same for every candidate; the last launch
+// below, with the chosen value, is the one whose volumes are kept).
+constexpr int kChaseCandidates[] = {0, 2, 4, 8, 16, 32};
+
+int launch_origin(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on, float* ms) {
   HIPCHK(c, hipMemsetAsync(c->y + (size_t)v0 * c->E, 0, sizeof(double) * (size_t)Vn * c->E, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 8, c->stream));
   if (c->Z > 0) {
     OriginArgs A = make_origin_args(c, var_on);
     A.fww += (size_t)v0 * c->E; A.rvw += (size_t)v0 * c->E; A.y += (size_t)v0 * c->E;
     if (var_on) A.var_on = var_on + v0;
-    if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    if (ms) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     hipLaunchKernelGGL(k_origin_lds, dim3(c->Z, Vn), dim3(kLdsBlock), c->lds_bytes, c->stream, A, 1);
-    if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    if (ms) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     HIPCHK(c, hipGetLastError());
+    if (ms) {
+      HIPCHK(c, hipEventSynchronize(c->ev[1]));
+      HIPCHK(c, hipEventElapsedTime(ms, c->ev[0], c->ev[1]));
+    }
+  }
+  return TAPGPU_OK;
+}
+
+int tune_chase(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
+  if (!c->ev[0])
+    for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
+  float best_ms = 0.f;
+  int best = kDefaultChase;
+  for (int k : kChaseCandidates) {
+    c->chase = k;
+    float ms = 0.f;
+    int rc = launch_origin(c, v0, Vn, var_on, &ms);
+    if (rc) return rc;
+    int herr = 0;
+    HIPCHK(c, hipMemcpyAsync(&herr, c->err_dev, sizeof herr, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (herr) break;                    // reported by the caller's launch
+    if (best_ms == 0.f || ms < best_ms) { best_ms = ms; best = k; }
+  }
+  c->chase = best;
+  c->stats.origin_tune_ms += best_ms;
+  return TAPGPU_OK;
+}
+
+int aon_lds(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
+  auto t0 = std::chrono::steady_clock::now();
+  HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
+  if (c->chase < 0 && c->Z > 0) {
+    int rc = tune_chase(c, v0, Vn, var_on);
+    if (rc) return rc;
+    HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
+  }
+  HIPCHK(c, hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 16, c->stream));
+  if (c->Z > 0) {
+    if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    int rc = launch_origin(c, v0, Vn, var_on, nullptr);
+    if (rc) return rc;
+    if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
   }
   int herr[3];
-  unsigned long long hc[8];
+  unsigned long long hc[16];
   HIPCHK(c, hipMemcpyAsync(herr, c->err_dev, sizeof herr, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(hc, c->counters, sizeof hc, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1202,6 +1262,12 @@ int aon_lds(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
   c->stats.origin_relax_rounds += (int64_t)hc[3];
   c->stats.origin_load_rounds += (int64_t)hc[4];
   c->stats.origin_loaded += (int64_t)hc[5];
+  c->stats.origin_walk_steps += (int64_t)hc[6];
+  if (c->clock_khz > 0) {
+    c->stats.origin_relax_ms += (double)hc[8] / c->clock_khz;
+    c->stats.origin_pred_ms += (double)hc[9] / c->clock_khz;
+    c->stats.origin_load_ms += (double)hc[10] / c->clock_khz;
+  }
   if (c->profiling && c->Z > 0) {
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
@@ -1209,6 +1275,7 @@ int aon_lds(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
     c->stats.origin_launches += 1;
   }
   c->stats.origin_blocks += (int64_t)c->Z * Vn;
+  c->stats.origin_chase = c->chase;
   c->stats.aon_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (herr[0] == 1)
     return c->fail(TAPGPU_E_NOPATH, "no path from %d to %d", herr[1] + 1, herr[2] + 1);
@@ -1398,18 +1465,17 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
     }
     h_od_start[o] = (long long)h_od_dest.size();
   }
-  // one-origin-per-workgroup path: labels must fit LDS, links 16-bit,
-  // out-degree within the 8-bit child counters
+  // one-origin-per-workgroup path: labels must fit LDS, link ids and CSR
+  // offsets 16-bit
   {
-    int maxdeg = 0;
-    for (int64_t v = 0; v < N; ++v) maxdeg = std::max(maxdeg, fw_off[v + 1] - fw_off[v]);
     int dev = 0, lds_max = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
       lds_max = 0;
     c->lds_bytes = origin_lds_bytes(c->N);
-    c->lds_ok = c->lds_bytes + 128 <= (size_t)lds_max && E < 65535 && maxdeg <= 255 &&
-                N <= kLdsMaxNodes;
+    if (hipDeviceGetAttribute(&c->clock_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess)
+      c->clock_khz = 0;
+    c->lds_ok = c->lds_bytes + 128 <= (size_t)lds_max && E < 65535 && N <= kLdsMaxNodes;
   }
 
   // --- link records per variant (capacity fix-up per variant) ---
@@ -1451,7 +1517,7 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
   ALLOC(col_src, c->P); ALLOC(grp_var, c->Galloc);
   c->flags_cap = (int)std::min<int64_t>(std::max<int64_t>(4 * N + 64, 1024), 1 << 24);
   ALLOC(flags, c->flags_cap);
-  ALLOC(counters, 8);
+  ALLOC(counters, 16);
 
   ALLOC(err_dev, 4);
   ALLOC(var_on, V); ALLOC(st, V);
@@ -1905,6 +1971,17 @@ int tapgpu_set_strategy(tapgpu_ctx* c, int strategy) {
     return c->fail(TAPGPU_E_ARG, "network does not fit the one-origin-per-workgroup path");
   c->strategy = strategy;
   return TAPGPU_OK;
+}
+
+int tapgpu_set_origin_chase(tapgpu_ctx* c, int chase) {
+  if (!c || chase < -1 || chase > 1024) return TAPGPU_E_ARG;
+  c->chase = chase;
+  return TAPGPU_OK;
+}
+
+int tapgpu_get_origin_chase(const tapgpu_ctx* c) {
+  if (!c) return TAPGPU_E_ARG;
+  return c->chase;
 }
 
 int tapgpu_get_strategy(const tapgpu_ctx* c) {

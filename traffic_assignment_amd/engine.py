@@ -66,7 +66,10 @@ class _Stats(C.Structure):
                 ("origin_blocks", C.c_int64), ("origin_expansions", C.c_int64),
                 ("origin_scanned", C.c_int64), ("origin_updates", C.c_int64),
                 ("origin_relax_rounds", C.c_int64), ("origin_load_rounds", C.c_int64),
-                ("origin_loaded", C.c_int64)]
+                ("origin_loaded", C.c_int64), ("origin_walk_steps", C.c_int64),
+                ("origin_relax_ms", C.c_double), ("origin_pred_ms", C.c_double),
+                ("origin_load_ms", C.c_double), ("origin_tune_ms", C.c_double),
+                ("origin_chase", C.c_int64)]
 
 
 class _GradInfo(C.Structure):
@@ -106,6 +109,8 @@ def lib():
         L.tapgpu_set_profiling.argtypes = [vp, C.c_int]
         L.tapgpu_set_strategy.argtypes = [vp, C.c_int]
         L.tapgpu_get_strategy.argtypes = [vp]
+        L.tapgpu_set_origin_chase.argtypes = [vp, C.c_int]
+        L.tapgpu_get_origin_chase.argtypes = [vp]
         L.tapgpu_gradient.argtypes = [vp, C.c_int64, C.c_double, C.c_double, C.c_int,
                                       vp, vp, vp, vp, C.POINTER(_GradInfo)]
         _lib = L
@@ -260,6 +265,15 @@ class TapEngine:
     @property
     def strategy(self):
         return {1: "sweeps", 2: "lds"}[lib().tapgpu_get_strategy(self._h)]
+
+    def set_origin_chase(self, chase):
+        """Chained expansions per thread and round on the one-origin-per-
+        workgroup path (-1: tune on the next loading)."""
+        self._check(lib().tapgpu_set_origin_chase(self._h, int(chase)))
+
+    @property
+    def origin_chase(self):
+        return lib().tapgpu_get_origin_chase(self._h)
 
     def set_profiling(self, on=True):
         self._check(lib().tapgpu_set_profiling(self._h, int(bool(on))))
