@@ -70,11 +70,12 @@ def origin_bytes(st, num_links) -> float:
 
 def cpu_baseline(net, dem, gpu_iters: int, cores: int):
     """The reference's tap_frankwolfe_pthread (compiled from its sources into
-    oracle/_ref) on this host: time 4 and 24 iterations, per-iteration time x
+    oracle/_ref) on this host: time lo = 2 iterations and hi = lo + k, k
+    chosen for about 20 s of CPU work (2 <= k <= 20); per-iteration time x
     the GPU's iteration count + setup = one solve (extrapolated)."""
     from traffic_assignment_amd import tntp
     ref = os.path.join(ROOT, "oracle", "_ref", "tap_frankwolfe_pthread")
-    lo, hi = 4, 24
+    lo = 2
     with tempfile.TemporaryDirectory() as td:
         nf, tf = os.path.join(td, "net.txt"), os.path.join(td, "trips.txt")
         tntp.write_net(nf, net)
@@ -95,7 +96,9 @@ def cpu_baseline(net, dem, gpu_iters: int, cores: int):
                 t = time.perf_counter()
                 oracle.fw_solve(net, dem, max_iters=it, nthreads=cores)
                 return time.perf_counter() - t
-        t_lo, t_hi = run(lo), run(hi)
+        t_lo = run(lo)
+        hi = lo + int(min(20, max(2, 20.0 / max(t_lo / lo, 1e-6))))
+        t_hi = run(hi)
     per_iter = (t_hi - t_lo) / (hi - lo)
     setup = max(t_lo - lo * per_iter, 0.0)
     solve = setup + per_iter * gpu_iters
@@ -264,7 +267,7 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=8,
                     help="batched: variants solved one by one for the plain comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--strategy", default="auto", choices=["auto", "sweeps", "lds"])
+    ap.add_argument("--strategy", default="auto", choices=["auto", "sweeps", "lds", "hbm"])
     ap.add_argument("--max-iters", type=int, default=0,
                     help="cap FW iterations per step (profiling runs; 0 = solve to 1e-4)")
     args = ap.parse_args()
@@ -308,7 +311,7 @@ def main():
 
     value = args.steps / elapsed
     if acc.get("origin_launches", 0) > 0:
-        kernel = "k_origin_lds"
+        kernel = "k_origin_hbm" if eng_path == "hbm" else "k_origin_lds"
         launches = acc["origin_launches"]
         avg_ms = acc["origin_kernel_ms"] / launches
         bytes_per_launch = origin_bytes(acc, net.num_links) / launches
@@ -337,6 +340,11 @@ def main():
                    "iterations": iters, "nodes": net.num_nodes, "links": net.num_links,
                    "zones": net.num_zones, "parallelism": "1 GPU"},
         "sssp_gbps": round(achieved, 2),
+        "schedule": {"origin_chase": acc.get("origin_chase", 0) // max(args.steps, 1),
+                     "origin_delta": acc.get("origin_delta", 0.0) / max(args.steps, 1)},
+        "origin_phase_us_per_workgroup": {
+            k: round(1e3 * acc.get("origin_" + k + "_ms", 0.0) / max(acc.get("origin_blocks", 0), 1), 2)
+            for k in ("relax", "pred", "load")},
         "phase_ms_per_step": {k: round(acc[k] / args.steps, 3) for k in
                               ("sssp_ms", "aon_ms", "link_ms", "total_ms")},
         "roofline": roof,

@@ -195,6 +195,8 @@ typedef struct {
   double origin_load_ms;
   double origin_tune_ms;        /* fastest launch of the schedule tuning     */
   int64_t origin_chase;         /* chained expansions per thread and round   */
+  double origin_delta;          /* bucket width of the labels-in-HBM kernel, */
+                                /*   in mean free-flow link times (0: none)  */
 } tapgpu_stats;
 
 /* Counters and wall times of the last tapgpu_fw_run (or of the calls since
@@ -204,11 +206,12 @@ int tapgpu_get_stats(tapgpu_ctx *ctx, tapgpu_stats *stats);
 /* on != 0: time every sweep / level launch with HIP events (small overhead). */
 int tapgpu_set_profiling(tapgpu_ctx *ctx, int on);
 
-/* Shortest-path / loading strategy: 0 = automatic (default), 1 = node-major
-   multi-origin sweeps (any size), 2 = one origin per workgroup with labels
-   in LDS (networks whose labels fit a workgroup's LDS: about 13k nodes,
-   fewer than 65535 links).  tapgpu_get_strategy returns the one in use
-   (1 or 2). */
+/* Shortest-path / loading strategy: 0 = automatic (default: 2 where it
+   fits, else 3), 1 = node-major multi-origin sweeps (any size), 2 = one
+   origin per workgroup with labels in LDS (networks whose labels fit a
+   workgroup's LDS: about 13k nodes, fewer than 65535 links), 3 = one origin
+   per workgroup with labels in HBM (frontier bitmaps in LDS: up to about
+   430k nodes).  tapgpu_get_strategy returns the one in use (1, 2 or 3). */
 int tapgpu_set_strategy(tapgpu_ctx *ctx, int strategy);
 
 /* Worklist schedule of the one-origin-per-workgroup path: how many labels a
@@ -218,6 +221,15 @@ int tapgpu_set_strategy(tapgpu_ctx *ctx, int strategy);
    (0, 2, 4, 8, 16, 32) on the real launch.  Results do not depend on it. */
 int tapgpu_set_origin_chase(tapgpu_ctx *ctx, int chase);
 int tapgpu_get_origin_chase(const tapgpu_ctx *ctx);
+
+/* Worklist schedule of the labels-in-HBM path (strategy 3): bucket width of
+   its label-ordered rounds, as a multiple of the mean free-flow link time
+   (0: no buckets, every queued node is expanded each round).  -1 (the
+   default): tuned on the first loading by timing each candidate
+   (0, 1, 2, 4, 8, 16) on one wave of workgroups.  Results do not depend
+   on it. */
+int tapgpu_set_origin_delta(tapgpu_ctx *ctx, double mult);
+double tapgpu_get_origin_delta(const tapgpu_ctx *ctx);
 int tapgpu_get_strategy(const tapgpu_ctx *ctx);
 
 /* --- equilibrium gradients ------------------------------------------------ */

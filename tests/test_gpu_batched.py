@@ -17,7 +17,7 @@ def _small():
     return base, dem, mods, subsets
 
 
-@pytest.mark.parametrize("strategy", ["sweeps", "lds"])
+@pytest.mark.parametrize("strategy", ["sweeps", "lds", "hbm"])
 def test_batched_equals_plain_and_oracle(engine_mod, oracle_lib, strategy):
     print(f"case batched {strategy}", flush=True)
     base, dem, mods, subsets = _small()

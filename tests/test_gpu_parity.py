@@ -13,7 +13,7 @@ from conftest import b64_array, empty_demand, golden, hexarr, network, pred_tie_
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-9
-STRATS = ["sweeps", "lds"]
+STRATS = ["sweeps", "lds", "hbm"]
 
 
 def make_engine(engine_mod, net, dem, strategy, **kw):
@@ -46,7 +46,7 @@ def _sssp_case(engine_mod, oracle_lib, net, dem, costs, origins, strategy="sweep
 
 
 @pytest.mark.parametrize("strategy", STRATS)
-@pytest.mark.parametrize("name,scale", [("grid5", 1.0), ("metro", 0.3)])
+@pytest.mark.parametrize("name,scale", [("grid5", 1.0), ("metro", 0.3), ("regional", 0.25)])
 def test_sssp_synthetic(engine_mod, oracle_lib, name, scale, strategy):
     print(f"case sssp {name}@{scale} {strategy}", flush=True)
     net, dem = network(name, scale)
@@ -105,7 +105,8 @@ def test_sssp_unreachable_and_bad_args(engine_mod):
 
 
 @pytest.mark.parametrize("strategy", STRATS)
-@pytest.mark.parametrize("name,scale", [("grid5", 1.0), ("metro", 0.3), ("metro", 1.0)])
+@pytest.mark.parametrize("name,scale", [("grid5", 1.0), ("metro", 0.3), ("metro", 1.0),
+                                        ("regional", 0.25)])
 def test_aon_exact(engine_mod, oracle_lib, name, scale, strategy):
     print(f"case aon {name}@{scale} {strategy}", flush=True)
     net, dem = network(name, scale)
@@ -186,6 +187,37 @@ def test_fw_relgap_target_and_resume(engine_mod, oracle_lib, strategy):
     assert r1["iterations"] == r2["iterations"] and np.array_equal(x1, x2)
     o = oracle_lib.fw_solve(net, dem, target_relgap=1e-4, nthreads=8)
     assert o["iterations"] == r1["iterations"]
+
+
+@pytest.mark.parametrize("strategy", ["hbm", "sweeps"])
+def test_fw_regional_first_iterations(engine_mod, oracle_lib, strategy):
+    """The regional network (94k nodes, 4,000 zones; too large for LDS
+    labels): two FW iterations bit-identical to the oracle (correctly
+    rounded x^4 in both)."""
+    print(f"case fw regional {strategy}", flush=True)
+    net, dem = network("regional")
+    with make_engine(engine_mod, net, dem, strategy) as eng:
+        eng.fw_init()
+        eng.fw_run(max_iters=2)
+        x, c = eng.flows()
+        tr = eng.trace()
+    o = _regional_oracle(oracle_lib)
+    assert np.array_equal(tr["lambda_"], o["trace"]["lambda_"])
+    assert np.array_equal(x, o["x"]) and np.array_equal(c, o["c"])
+
+
+_REGIONAL = {}
+
+
+def _regional_oracle(oracle_lib):
+    if not _REGIONAL:
+        net, dem = network("regional")
+        oracle_lib.lib().ora_set_pow_mode(1)
+        try:
+            _REGIONAL["o"] = oracle_lib.fw_solve(net, dem, max_iters=2, nthreads=16)
+        finally:
+            oracle_lib.lib().ora_set_pow_mode(0)
+    return _REGIONAL["o"]
 
 
 def test_fw_warm_start_continues(engine_mod):

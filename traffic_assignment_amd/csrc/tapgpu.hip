@@ -436,6 +436,7 @@ struct OriginArgs {
   const double* od_val;
   const unsigned char* var_on;       // [V] or null
   int chase;                         // expansions a thread chains per round
+  double delta;                      // labels-in-HBM kernel: bucket width (<= 0: none)
   double* y;                         // [V * E], zeroed by the caller
   int* err;                          // [3]
   unsigned long long* counters;      // [16] expansions, links scanned, updates,
@@ -448,6 +449,11 @@ struct OriginArgs {
   double* out_dist;                  // [(N) * ld] node-major, row v = node v+1
   int* out_pred;
   long long ld;
+  // labels-in-HBM kernel: per-block scratch rows of N entries
+  unsigned long long* h_dist;        // labels, then flows
+  unsigned* h_rnd;                   // round of the last lowering
+  int* h_pred;                       // tree link
+  int* h_ptail;                      // its tail
 };
 
 __device__ __forceinline__ double as_d(unsigned long long b) { return __longlong_as_double((long long)b); }
@@ -682,6 +688,252 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
     atomicAdd(&A.counters[8], (unsigned long long)(clk1 - clk0));
     atomicAdd(&A.counters[9], (unsigned long long)(clk2 - clk1));
     atomicAdd(&A.counters[10], (unsigned long long)(clk3 - clk2));
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// one origin per workgroup, labels in HBM (networks of any size)
+// ---------------------------------------------------------------------------
+//
+// The same three phases as k_origin_lds for networks whose labels do not fit
+// LDS: only the three frontier bitmaps (3 x 8 x ceil(N/64) bytes) live in
+// LDS; labels, rounds, predecessors and flows are a per-workgroup scratch
+// row in HBM (L2 / infinity cache resident while the workgroup runs).  A
+// persistent grid loops over (origin, variant) items.  A row is private to
+// its workgroup, whose waves share one CU, so every scratch access is a
+// workgroup-scope atomic: loads skip the CU's L1 and read the XCD's L2,
+// where the atomics execute, and the barrier closing a round orders them.
+// (Device-scope accesses would have to be coherent across the XCDs' L2s and
+// cost an L2 writeback and invalidation per round.)
+// Relaxation within a round may read a label another thread is lowering;
+// the lowered node is queued for the next round, so the fixed point is the
+// same as with strict rounds.
+
+constexpr int kHbmBlock = 1024;
+
+// workgroup-scope loads of a workgroup's private scratch row
+__device__ __forceinline__ unsigned long long ldg_u64(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ unsigned ldg_u32(const unsigned* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ int ldg_i32(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__global__ __launch_bounds__(kHbmBlock) void k_origin_hbm(OriginArgs A, int load, int items) {
+  extern __shared__ unsigned long long sm[];
+  const int N = A.N;
+  const int W = (N + 63) >> 6;
+  unsigned long long* bmA = sm;
+  unsigned long long* bmB = bmA + W;
+  unsigned long long* bmC = bmB + W;
+  __shared__ int s_more[3], s_exp[3];
+  __shared__ unsigned long long s_minq[3];
+  __shared__ unsigned long long s_cnt[8];
+  __shared__ int s_maxlen;
+  const int tid = threadIdx.x;
+  const unsigned long long kMax = as_b(DBL_MAX);
+  unsigned long long* dist = A.h_dist + (size_t)blockIdx.x * N;
+  unsigned* rnd = A.h_rnd + (size_t)blockIdx.x * N;
+  int* pred = A.h_pred + (size_t)blockIdx.x * N;
+  int* ptail = A.h_ptail + (size_t)blockIdx.x * N;
+  double* flow = (double*)dist;
+  if (tid < 8) s_cnt[tid] = 0;
+  if (tid == 0) s_maxlen = 0;
+  unsigned long long n_exp = 0, n_scan = 0, n_upd = 0, n_rounds = 0, n_steps = 0, n_loaded = 0;
+  unsigned long long c_relax = 0, c_pred = 0, c_load = 0;
+
+  for (int item = blockIdx.x; item < items; item += gridDim.x) {
+    const int var = A.src_list ? 0 : item / A.Z;
+    const int src = A.src_list ? A.src_list[item] : item - var * A.Z;
+    if (A.var_on && !A.var_on[var]) continue;
+    const double* __restrict__ fw = A.fww + (size_t)var * A.E;
+    const double* __restrict__ rw = A.rvw + (size_t)var * A.E;
+    const long long clk0 = wall_clock64();
+    for (int v = tid; v < N; v += kHbmBlock) {
+      __hip_atomic_store(&dist[v], kMax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_store(&rnd[v], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    for (int w = tid; w < 3 * W; w += kHbmBlock) bmA[w] = 0;
+    if (tid < 3) { s_more[tid] = 0; s_exp[tid] = 0; s_minq[tid] = ~0ull; }
+    __syncthreads();
+    if (tid == 0) {
+      __hip_atomic_store(&dist[src], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      bmA[src >> 6] = 1ull << (src & 63);
+    }
+    __syncthreads();
+
+    // ---- label-correcting rounds, bucketed: a round expands the queued
+    //      nodes whose label is below T and keeps the others queued; a
+    //      round that expands nothing moves T to the smallest queued label
+    //      plus delta (delta <= 0: no buckets, every queued node expands) ----
+    unsigned long long *cur = bmA, *nxt = bmB, *clr = bmC;
+    const bool buckets = A.delta > 0.0;
+    double T = buckets ? A.delta : INFINITY;
+    int r = 1;
+    for (;; ++r) {
+      const int sl = r % 3;
+      for (int slot = tid; slot < 4 * W; slot += kHbmBlock) {
+        const int w = slot >> 2, q = slot & 3;
+        unsigned bits = (unsigned)((cur[w] >> (16 * q)) & 0xFFFFull);
+        while (bits) {
+          const int b = __ffs(bits) - 1;
+          bits &= bits - 1;
+          const int u = (w << 6) + 16 * q + b;
+          const unsigned long long dub = ldg_u64(&dist[u]);
+          const double du = as_d(dub);
+          if (!(du < T)) {                      // later bucket: stays queued
+            atomicOr(&nxt[u >> 6], 1ull << (u & 63));
+            atomicMin(&s_minq[sl], dub);
+            s_more[sl] = 1;
+            continue;
+          }
+          s_exp[sl] = 1;
+          const int eb = A.fw_off[u], ee = A.fw_off[u + 1];
+          ++n_exp;
+          n_scan += (unsigned long long)(ee - eb);
+          for (int e0 = eb; e0 < ee; e0 += 8) {
+            int hv[8];
+            double hw[8];
+            unsigned long long hd[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+              if (e0 + k < ee) { hv[k] = A.fw_dst[e0 + k]; hw[k] = fw[e0 + k]; }
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+              if (e0 + k < ee) hd[k] = ldg_u64(&dist[hv[k]]);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+              if (e0 + k >= ee) break;
+              const int v = hv[k];
+              const unsigned long long cb = as_b(du + hw[k]);
+              if (cb < hd[k]) {
+                const unsigned long long old =
+                    __hip_atomic_fetch_min(&dist[v], cb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (cb < old) {
+                  ++n_upd;
+                  __hip_atomic_store(&rnd[v], (unsigned)r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                  if (v >= A.ft0) {
+                    atomicOr(&nxt[v >> 6], 1ull << (v & 63));
+                    s_more[sl] = 1;
+                  }
+                }
+              }
+            }
+          }
+        }
+      }
+      for (int w = tid; w < W; w += kHbmBlock) clr[w] = 0;
+      if (tid == 0) {
+        const int sn = (r + 1) % 3;
+        s_more[sn] = 0;
+        s_exp[sn] = 0;
+        s_minq[sn] = ~0ull;
+      }
+        __syncthreads();
+      if (!s_more[sl]) break;
+      if (buckets && !s_exp[sl]) T = as_d(s_minq[sl]) + A.delta;
+      if (r > 64 * N + 64) {
+        if (tid == 0 && atomicCAS(&A.err[0], 0, 2) == 0) { A.err[1] = src; A.err[2] = 0; }
+        break;
+      }
+      unsigned long long* t = cur; cur = nxt; nxt = clr; clr = t;
+    }
+    n_rounds += (unsigned long long)r;
+    const long long clk1 = wall_clock64();
+
+    // ---- predecessor links: smallest tail label, then (round, node) ----
+    for (int v = tid; v < N; v += kHbmBlock) {
+      int p = -1, pt = -1;
+      const unsigned long long dv = ldg_u64(&dist[v]);
+      if (v != src && dv != kMax) {
+        unsigned long long best = ~0ull, best_du = ~0ull;
+        for (int k = A.rv_off[v]; k < A.rv_off[v + 1]; ++k) {
+          const int u = A.rv_src[k];
+          if (u == v || (u < A.ft0 && u != src)) continue;
+          const unsigned long long du = ldg_u64(&dist[u]);
+          if (du == kMax) continue;
+          if (as_b(as_d(du) + rw[k]) != dv) continue;
+          const unsigned long long key = ((unsigned long long)ldg_u32(&rnd[u]) << 32) | (unsigned)u;
+          if (du < best_du || (du == best_du && key < best)) {
+            best = key;
+            best_du = du;
+            p = A.rv_eid[k];
+            pt = u;
+          }
+        }
+      }
+      __hip_atomic_store(&pred[v], p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_store(&ptail[v], pt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (A.out_dist) {
+        const size_t o = (size_t)v * A.ld + item;
+        A.out_dist[o] = as_d(dv);
+        A.out_pred[o] = p;
+      }
+    }
+    __syncthreads();          // every label read is done: the row now holds flows
+    const long long clk2 = wall_clock64();
+    c_relax += (unsigned long long)(clk1 - clk0);
+    c_pred += (unsigned long long)(clk2 - clk1);
+    if (!load) continue;
+
+    // ---- loading: destination walks, then tree links with flow ----
+    for (int v = tid; v < N; v += kHbmBlock)
+      __hip_atomic_store(&dist[v], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);   // +0.0
+    __syncthreads();
+    int maxlen = 0;
+    for (long long i = A.od_start[src] + tid; i < A.od_start[src + 1]; i += kHbmBlock) {
+      const int d = A.od_dest[i];
+      const double f = A.od_val[i];
+      if (d == src || !(f > 0.0)) continue;
+      if (ldg_i32(&pred[d]) < 0) {
+        if (atomicCAS(&A.err[0], 0, 1) == 0) { A.err[1] = src; A.err[2] = d; }
+        continue;
+      }
+      int v = d, len = 0;
+      while (v != src) {
+        if (len > N || v < 0) {
+          if (atomicCAS(&A.err[0], 0, 3) == 0) { A.err[1] = src; A.err[2] = 0; }
+          break;
+        }
+        __hip_atomic_fetch_add(&flow[v], f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        v = ldg_i32(&ptail[v]);
+        ++len;
+      }
+      n_steps += (unsigned long long)len;
+      maxlen = max(maxlen, len);
+    }
+    if (maxlen) atomicMax(&s_maxlen, maxlen);
+    __syncthreads();
+    double* __restrict__ yv = A.y + (size_t)var * A.E;
+    for (int v = tid; v < N; v += kHbmBlock) {
+      const double f = as_d(ldg_u64(&dist[v]));
+      if (f != 0.0) {
+        const int p = ldg_i32(&pred[v]);
+        if (p >= 0) {
+          unsafeAtomicAdd(&yv[p], f);
+          ++n_loaded;
+        }
+      }
+    }
+    __syncthreads();          // the row is reused by the next item
+    c_load += (unsigned long long)(wall_clock64() - clk2);
+  }
+  if (A.counters) {
+    atomicAdd(&s_cnt[0], n_exp); atomicAdd(&s_cnt[1], n_scan); atomicAdd(&s_cnt[2], n_upd);
+    atomicAdd(&s_cnt[5], n_loaded); atomicAdd(&s_cnt[6], n_steps);
+    __syncthreads();
+    if (tid == 0) {
+      atomicAdd(&A.counters[0], s_cnt[0]); atomicAdd(&A.counters[1], s_cnt[1]);
+      atomicAdd(&A.counters[2], s_cnt[2]); atomicAdd(&A.counters[3], n_rounds);
+      atomicAdd(&A.counters[4], (unsigned long long)s_maxlen);
+      atomicAdd(&A.counters[5], s_cnt[5]); atomicAdd(&A.counters[6], s_cnt[6]);
+      atomicAdd(&A.counters[8], c_relax); atomicAdd(&A.counters[9], c_pred);
+      atomicAdd(&A.counters[10], c_load);
+    }
   }
 }
 
@@ -922,6 +1174,14 @@ struct tapgpu_ctx {
   // strategy: 0 auto, 1 node-major sweeps, 2 one origin per workgroup (LDS)
   int strategy = 0;
   int chase = -1;                     // LDS path: chained expansions (-1: tune)
+  bool hbm_ok = false;                // labels-in-HBM path: bitmaps fit LDS
+  size_t hbm_lds = 0;
+  int hbm_grid = 0;                   // persistent workgroups
+  double delta_mult = -1;             // bucket width / mean free-flow time (-1: tune)
+  double mean_t0 = 1;                 // mean free-flow time of variant 0's links
+  unsigned long long* h_dist = nullptr;
+  unsigned* h_rnd = nullptr;
+  int *h_pred = nullptr, *h_ptail = nullptr;
   int clock_khz = 0;                  // device wall clock
   bool labels_ready = false;          // node-major buffers allocated
   std::vector<double> h_odc;          // demand by (zone row, column), host copy
@@ -1177,6 +1437,11 @@ bool use_lds(tapgpu_ctx* c) {
   return c->lds_ok && (c->strategy == 0 || c->strategy == 2);
 }
 
+// auto: labels in LDS where they fit, else labels in HBM
+bool use_hbm(tapgpu_ctx* c) {
+  return c->hbm_ok && (c->strategy == 3 || (c->strategy == 0 && !c->lds_ok));
+}
+
 OriginArgs make_origin_args(tapgpu_ctx* c, const unsigned char* var_on) {
   OriginArgs A{};
   A.N = c->N; A.E = c->E; A.ft0 = c->ft0; A.Z = c->Z;
@@ -1185,6 +1450,8 @@ OriginArgs make_origin_args(tapgpu_ctx* c, const unsigned char* var_on) {
   A.od_start = c->od_start; A.od_dest = c->od_dest; A.od_val = c->od_val;
   A.var_on = var_on; A.y = c->y; A.err = c->err_dev; A.counters = c->counters;
   A.chase = c->chase >= 0 ? c->chase : kDefaultChase;
+  A.h_dist = c->h_dist; A.h_rnd = c->h_rnd; A.h_pred = c->h_pred; A.h_ptail = c->h_ptail;
+  A.delta = c->delta_mult > 0 ? c->delta_mult * c->mean_t0 : 0.0;
   return A;
 }
 
@@ -1236,21 +1503,8 @@ int tune_chase(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
   return TAPGPU_OK;
 }
 
-int aon_lds(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
-  auto t0 = std::chrono::steady_clock::now();
-  HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
-  if (c->chase < 0 && c->Z > 0) {
-    int rc = tune_chase(c, v0, Vn, var_on);
-    if (rc) return rc;
-    HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
-  }
-  HIPCHK(c, hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 16, c->stream));
-  if (c->Z > 0) {
-    if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    int rc = launch_origin(c, v0, Vn, var_on, nullptr);
-    if (rc) return rc;
-    if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-  }
+// Reads the error flag and counters of an origin-kernel launch into stats.
+int finish_origin(tapgpu_ctx* c, int Vn, std::chrono::steady_clock::time_point t0) {
   int herr[3];
   unsigned long long hc[16];
   HIPCHK(c, hipMemcpyAsync(herr, c->err_dev, sizeof herr, hipMemcpyDeviceToHost, c->stream));
@@ -1287,10 +1541,107 @@ int aon_lds(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
   return TAPGPU_OK;
 }
 
+int aon_lds(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
+  auto t0 = std::chrono::steady_clock::now();
+  HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
+  if (c->chase < 0 && c->Z > 0) {
+    int rc = tune_chase(c, v0, Vn, var_on);
+    if (rc) return rc;
+    HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
+  }
+  HIPCHK(c, hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 16, c->stream));
+  if (c->Z > 0) {
+    if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    int rc = launch_origin(c, v0, Vn, var_on, nullptr);
+    if (rc) return rc;
+    if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  }
+  return finish_origin(c, Vn, t0);
+}
+
+// Per-workgroup scratch rows of the labels-in-HBM kernel (allocated on
+// first use: hbm_grid rows of N labels, rounds, predecessors and tails).
+int ensure_hbm(tapgpu_ctx* c) {
+  if (c->h_dist) return TAPGPU_OK;
+  const size_t n = (size_t)c->hbm_grid * c->N;
+  int rc;
+  if ((rc = dalloc(c, &c->h_dist, n))) return rc;
+  if ((rc = dalloc(c, &c->h_rnd, n))) return rc;
+  if ((rc = dalloc(c, &c->h_pred, n))) return rc;
+  if ((rc = dalloc(c, &c->h_ptail, n))) return rc;
+  return TAPGPU_OK;
+}
+
+// Shortest paths + loading of variants [v0, v0+Vn) with the labels-in-HBM
+// kernel (persistent grid over the Z * Vn (origin, variant) items).
+//
+// Bucket-width tuning: the first loading on a context with delta_mult < 0
+// times one wave of workgroups (the first hbm_grid items) for each
+// candidate width, in units of the mean free-flow link time, and keeps the
+// fastest; the full launch follows.  Results do not depend on the width.
+constexpr double kDeltaCandidates[] = {0.0, 1.0, 2.0, 4.0, 8.0, 16.0};
+
+int launch_hbm(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on, int items_cap,
+               float* ms) {
+  HIPCHK(c, hipMemsetAsync(c->y + (size_t)v0 * c->E, 0, sizeof(double) * (size_t)Vn * c->E, c->stream));
+  if (c->Z == 0) return TAPGPU_OK;
+  OriginArgs A = make_origin_args(c, var_on);
+  A.fww += (size_t)v0 * c->E; A.rvw += (size_t)v0 * c->E; A.y += (size_t)v0 * c->E;
+  if (var_on) A.var_on = var_on + v0;
+  const int items = std::min(c->Z * Vn, items_cap);
+  const int G = std::min(c->hbm_grid, items);
+  if (ms) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  hipLaunchKernelGGL(k_origin_hbm, dim3(G), dim3(kHbmBlock), c->hbm_lds, c->stream, A, 1, items);
+  HIPCHK(c, hipGetLastError());
+  if (ms) {
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev[1]));
+    HIPCHK(c, hipEventElapsedTime(ms, c->ev[0], c->ev[1]));
+  }
+  return TAPGPU_OK;
+}
+
+int tune_delta(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
+  if (!c->ev[0])
+    for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
+  float best_ms = 0.f;
+  double best = 4.0;
+  for (double m : kDeltaCandidates) {
+    c->delta_mult = m;
+    float ms = 0.f;
+    int rc = launch_hbm(c, v0, Vn, var_on, c->hbm_grid, &ms);
+    if (rc) return rc;
+    int herr = 0;
+    HIPCHK(c, hipMemcpyAsync(&herr, c->err_dev, sizeof herr, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (herr) break;
+    if (best_ms == 0.f || ms < best_ms) { best_ms = ms; best = m; }
+  }
+  c->delta_mult = best;
+  c->stats.origin_tune_ms += best_ms;
+  HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
+  return TAPGPU_OK;
+}
+
+int aon_hbm(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
+  auto t0 = std::chrono::steady_clock::now();
+  int rc = ensure_hbm(c);
+  if (rc) return rc;
+  HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
+  if (c->delta_mult < 0 && c->Z > 0 && (rc = tune_delta(c, v0, Vn, var_on))) return rc;
+  HIPCHK(c, hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 16, c->stream));
+  if (c->profiling && c->Z > 0) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  if ((rc = launch_hbm(c, v0, Vn, var_on, INT32_MAX, nullptr))) return rc;
+  if (c->profiling && c->Z > 0) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  c->stats.origin_delta = c->delta_mult;
+  return finish_origin(c, Vn, t0);
+}
+
 // All-or-nothing loading of variants [v0, v0 + Vn) (groups of those
 // variants) with the current costs, by the configured strategy.
 int load_variants(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
   if (use_lds(c)) return aon_lds(c, v0, Vn, var_on);
+  if (use_hbm(c)) return aon_hbm(c, v0, Vn, var_on);
   int rc = sssp_groups(c, v0 * c->Gv, Vn * c->Gv, c->col_src, c->grp_var, var_on);
   if (!rc) rc = aon_groups(c, v0 * c->Gv, Vn * c->Gv, v0, Vn, var_on);
   return rc;
@@ -1476,6 +1827,22 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
     if (hipDeviceGetAttribute(&c->clock_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess)
       c->clock_khz = 0;
     c->lds_ok = c->lds_bytes + 128 <= (size_t)lds_max && E < 65535 && N <= kLdsMaxNodes;
+    c->hbm_lds = 24ull * ((size_t)(N + 63) / 64);
+    c->hbm_ok = c->hbm_lds + 256 <= (size_t)lds_max;
+    int ncu = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) ncu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_origin_hbm, kHbmBlock, c->hbm_lds) !=
+        hipSuccess || per_cu < 1)
+      per_cu = 1;
+    c->hbm_grid = std::max(1, ncu * per_cu);
+    double sum = 0;
+    int cnt = 0;
+    for (int64_t k = 0; k < E; ++k) {
+      const double t = (t0_v ? t0_v[k] : net->free_flow_time[k]);
+      const bool on = !present || present[k];
+      if (on && t > 0 && std::isfinite(t)) { sum += t; ++cnt; }
+    }
+    c->mean_t0 = cnt ? sum / cnt : 1.0;
   }
 
   // --- link records per variant (capacity fix-up per variant) ---
@@ -1641,8 +2008,11 @@ int tapgpu_sssp(tapgpu_ctx* c, int64_t variant, const int64_t* start_nodes, int6
     if (dist) dist[col_base + i] = DBL_MAX;
     if (pred) pred[col_base + i] = -1;
   }
-  if (use_lds(c)) {
-    // one workgroup per start node; chunks of up to 1024 starts
+  if (use_lds(c) || use_hbm(c)) {
+    // one workgroup per start node (labels in LDS), or the persistent
+    // labels-in-HBM grid; chunks of up to 1024 starts
+    const bool hbm = !use_lds(c);
+    if (hbm && (rc = ensure_hbm(c))) return rc;
     const int64_t chunk = std::min<int64_t>(std::max<int64_t>(num_start, 1), 1024);
     int* d_src = nullptr;
     double* d_dist = nullptr;
@@ -1660,8 +2030,12 @@ int tapgpu_sssp(tapgpu_ctx* c, int64_t variant, const int64_t* start_nodes, int6
       OriginArgs A = make_origin_args(c, nullptr);
       A.fww += (size_t)variant * c->E; A.rvw += (size_t)variant * c->E;
       A.src_list = d_src; A.out_dist = d_dist; A.out_pred = d_pred; A.ld = chunk;
-      hipLaunchKernelGGL(k_origin_lds, dim3((unsigned)ns, 1), dim3(kLdsBlock), c->lds_bytes,
-                         c->stream, A, 0);
+      if (hbm)
+        hipLaunchKernelGGL(k_origin_hbm, dim3((unsigned)std::min<int64_t>(ns, c->hbm_grid)),
+                           dim3(kHbmBlock), c->hbm_lds, c->stream, A, 0, (int)ns);
+      else
+        hipLaunchKernelGGL(k_origin_lds, dim3((unsigned)ns, 1), dim3(kLdsBlock), c->lds_bytes,
+                           c->stream, A, 0);
       const hipError_t e0 = hipGetLastError();
       const hipError_t e1 = hipMemcpyAsync(hd.data(), d_dist, sizeof(double) * c->N * chunk,
                                            hipMemcpyDeviceToHost, c->stream);
@@ -1966,9 +2340,11 @@ int tapgpu_get_stats(tapgpu_ctx* c, tapgpu_stats* s) {
 }
 
 int tapgpu_set_strategy(tapgpu_ctx* c, int strategy) {
-  if (!c || strategy < 0 || strategy > 2) return TAPGPU_E_ARG;
+  if (!c || strategy < 0 || strategy > 3) return TAPGPU_E_ARG;
   if (strategy == 2 && !c->lds_ok)
     return c->fail(TAPGPU_E_ARG, "network does not fit the one-origin-per-workgroup path");
+  if (strategy == 3 && !c->hbm_ok)
+    return c->fail(TAPGPU_E_ARG, "frontier bitmaps of this network do not fit LDS");
   c->strategy = strategy;
   return TAPGPU_OK;
 }
@@ -1979,6 +2355,14 @@ int tapgpu_set_origin_chase(tapgpu_ctx* c, int chase) {
   return TAPGPU_OK;
 }
 
+int tapgpu_set_origin_delta(tapgpu_ctx* c, double mult) {
+  if (!c || !(mult >= -1.0) || !std::isfinite(mult)) return TAPGPU_E_ARG;
+  c->delta_mult = mult < 0 ? -1.0 : mult;
+  return TAPGPU_OK;
+}
+
+double tapgpu_get_origin_delta(const tapgpu_ctx* c) { return c ? c->delta_mult : -2.0; }
+
 int tapgpu_get_origin_chase(const tapgpu_ctx* c) {
   if (!c) return TAPGPU_E_ARG;
   return c->chase;
@@ -1986,7 +2370,8 @@ int tapgpu_get_origin_chase(const tapgpu_ctx* c) {
 
 int tapgpu_get_strategy(const tapgpu_ctx* c) {
   if (!c) return TAPGPU_E_ARG;
-  return (c->lds_ok && c->strategy != 1) ? 2 : 1;
+  tapgpu_ctx* m = const_cast<tapgpu_ctx*>(c);
+  return use_lds(m) ? 2 : (use_hbm(m) ? 3 : 1);
 }
 
 int tapgpu_set_profiling(tapgpu_ctx* c, int on) {

@@ -69,7 +69,7 @@ class _Stats(C.Structure):
                 ("origin_loaded", C.c_int64), ("origin_walk_steps", C.c_int64),
                 ("origin_relax_ms", C.c_double), ("origin_pred_ms", C.c_double),
                 ("origin_load_ms", C.c_double), ("origin_tune_ms", C.c_double),
-                ("origin_chase", C.c_int64)]
+                ("origin_chase", C.c_int64), ("origin_delta", C.c_double)]
 
 
 class _GradInfo(C.Structure):
@@ -111,6 +111,9 @@ def lib():
         L.tapgpu_get_strategy.argtypes = [vp]
         L.tapgpu_set_origin_chase.argtypes = [vp, C.c_int]
         L.tapgpu_get_origin_chase.argtypes = [vp]
+        L.tapgpu_set_origin_delta.argtypes = [vp, C.c_double]
+        L.tapgpu_get_origin_delta.argtypes = [vp]
+        L.tapgpu_get_origin_delta.restype = C.c_double
         L.tapgpu_gradient.argtypes = [vp, C.c_int64, C.c_double, C.c_double, C.c_int,
                                       vp, vp, vp, vp, C.POINTER(_GradInfo)]
         _lib = L
@@ -257,14 +260,14 @@ class TapEngine:
         out["info"] = {f: getattr(info, f) for f, _ in _GradInfo._fields_ if f != "reserved"}
         return out
 
-    STRATEGIES = {"auto": 0, "sweeps": 1, "lds": 2}
+    STRATEGIES = {"auto": 0, "sweeps": 1, "lds": 2, "hbm": 3}
 
     def set_strategy(self, name):
         self._check(lib().tapgpu_set_strategy(self._h, self.STRATEGIES[name]))
 
     @property
     def strategy(self):
-        return {1: "sweeps", 2: "lds"}[lib().tapgpu_get_strategy(self._h)]
+        return {1: "sweeps", 2: "lds", 3: "hbm"}[lib().tapgpu_get_strategy(self._h)]
 
     def set_origin_chase(self, chase):
         """Chained expansions per thread and round on the one-origin-per-
@@ -274,6 +277,15 @@ class TapEngine:
     @property
     def origin_chase(self):
         return lib().tapgpu_get_origin_chase(self._h)
+
+    def set_origin_delta(self, mult):
+        """Bucket width of the labels-in-HBM path in mean free-flow link
+        times (0: none, -1: tune on the next loading)."""
+        self._check(lib().tapgpu_set_origin_delta(self._h, float(mult)))
+
+    @property
+    def origin_delta(self):
+        return lib().tapgpu_get_origin_delta(self._h)
 
     def set_profiling(self, on=True):
         self._check(lib().tapgpu_set_profiling(self._h, int(bool(on))))
