@@ -268,6 +268,10 @@ def main():
                     help="batched: variants solved one by one for the plain comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--strategy", default="auto", choices=["auto", "sweeps", "lds", "hbm"])
+    ap.add_argument("--chase", type=int, default=-1,
+                    help="LDS path: chained expansions per round (-1: tuned)")
+    ap.add_argument("--delta", type=float, default=-1.0,
+                    help="HBM path: bucket width in mean free-flow times (-1: tuned)")
     ap.add_argument("--max-iters", type=int, default=0,
                     help="cap FW iterations per step (profiling runs; 0 = solve to 1e-4)")
     args = ap.parse_args()
@@ -290,6 +294,8 @@ def main():
     eng = TapEngine(net, dem)          # graph, link data and demand -> HBM
     if args.strategy != "auto":
         eng.set_strategy(args.strategy)
+    eng.set_origin_chase(args.chase)
+    eng.set_origin_delta(args.delta)
     for _ in range(args.warmup):
         eng.fw_init()
         eng.fw_run(max_iters=args.max_iters, target_relgap=TARGET)

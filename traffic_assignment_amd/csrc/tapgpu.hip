@@ -710,7 +710,7 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
 // the lowered node is queued for the next round, so the fixed point is the
 // same as with strict rounds.
 
-constexpr int kHbmBlock = 1024;
+constexpr int kHbmBlock = 256;                 // 4 workgroups per CU: more origins in flight
 
 // workgroup-scope loads of a workgroup's private scratch row
 __device__ __forceinline__ unsigned long long ldg_u64(const unsigned long long* p) {
