@@ -220,6 +220,33 @@ def _regional_oracle(oracle_lib):
     return _REGIONAL["o"]
 
 
+def test_schedule_knobs_do_not_change_results(engine_mod):
+    """Chained expansions (LDS kernel) and bucket widths (HBM kernel) are
+    schedules only: volumes bit-identical for every setting; bad values
+    rejected."""
+    print("case schedule knobs", flush=True)
+    net, dem = network("metro", 0.3)
+    out = []
+    with engine_mod.TapEngine(net, dem) as eng:
+        eng.set_strategy("lds")
+        for ch in (0, 1, 4, 32):
+            eng.set_origin_chase(ch)
+            out.append(eng.aon())
+        with pytest.raises(engine_mod.TapGpuError):
+            eng.set_origin_chase(-2)
+        eng.set_strategy("hbm")
+        for d in (0.0, 0.5, 4.0, 64.0):
+            eng.set_origin_delta(d)
+            out.append(eng.aon())
+        with pytest.raises(engine_mod.TapGpuError):
+            eng.set_origin_delta(float("nan"))
+        eng.set_origin_delta(-1)          # tune again on the next loading
+        out.append(eng.aon())
+        assert eng.origin_delta >= 0
+    for y in out[1:]:
+        assert np.array_equal(y, out[0])
+
+
 def test_fw_warm_start_continues(engine_mod):
     net, dem = network("metro", 0.3)
     with engine_mod.TapEngine(net, dem) as eng:
