@@ -136,11 +136,19 @@ def bench_batched(args):
     acc = {}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for step in range(args.steps):
         eng.fw_init()
-        res = eng.fw_run(target_relgap=TARGET)
-        for k, v in eng.stats().items():
-            acc[k] = acc.get(k, 0) + v
+        done = 0
+        while True:         # chunks of 25 iterations, a progress line each
+            done += 25
+            res = eng.fw_run(max_iters=done, target_relgap=TARGET)
+            for k, v in eng.stats().items():
+                acc[k] = acc.get(k, 0) + v
+            running = sum(1 for r in res if not r["converged"] and r["iterations"] >= done)
+            print(f"step {step}: {done} iterations, {running} variants running",
+                  file=sys.stderr, flush=True)
+            if running == 0:
+                break
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     path = eng.strategy
