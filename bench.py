@@ -204,21 +204,33 @@ def bench_batched(args):
 
 
 def bench_gradient(args):
-    """Gradient form on metro: one cold FW solve to relgap 1e-6, then --steps
-    timed gradient calls (envelope d Beckmann/d(c, t0) and implicit
-    d VHT/d(c, t0) for every link), against warm re-solves after a 1e-4
-    capacity change -- what one central finite difference per link costs."""
+    """Gradient form: one cold FW solve of --grad-network to relgap 1e-6,
+    then --steps timed gradient calls (envelope d Beckmann/d(c, t0) and
+    implicit d VHT/d(c, t0) for every link), against warm re-solves after a
+    1e-4 capacity change -- what one central finite difference per link
+    costs."""
     import torch
     from traffic_assignment_amd import synth
     from traffic_assignment_amd.engine import TapEngine
 
     target = 1e-6
-    net, dem = synth.make("metro")
+    net, dem = synth.make(args.grad_network)
+
+    def solve(eng):
+        done, r = 0, None
+        while True:         # chunks with a progress line each
+            done += 200
+            r = eng.fw_run(max_iters=min(done, 100000), target_relgap=target)[0]
+            print(f"{r['iterations']} iterations, relgap {r['relgap']:.3e}", file=sys.stderr,
+                  flush=True)
+            if r["converged"] or done >= 100000:
+                return r
+
     eng = TapEngine(net, dem)
     torch.cuda.synchronize()
     t = time.perf_counter()
     eng.fw_init()
-    r = eng.fw_run(max_iters=args.max_iters or 100000, target_relgap=target)[0]
+    r = solve(eng)
     torch.cuda.synchronize()
     solve_s = time.perf_counter() - t
     x, _ = eng.flows()
@@ -242,7 +254,7 @@ def bench_gradient(args):
             e2.fw_init(x)
             torch.cuda.synchronize()
             t = time.perf_counter()
-            e2.fw_run(max_iters=100000, target_relgap=target)
+            solve(e2)
             torch.cuda.synchronize()
             warm.append(time.perf_counter() - t)
     resolve = float(np.mean(warm))
@@ -253,8 +265,7 @@ def bench_gradient(args):
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * grad_s, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded generator, traffic_assignment_amd/synth.py)",
-        "config": {"workload": "gradient form on " + DESCR["metro"] + " at relgap 1e-6 (the "
-                               "regional network at 1e-6 needs ~1e4+ FW iterations of 0.5 s)",
+        "config": {"workload": "gradient form on " + DESCR[args.grad_network] + " at relgap 1e-6",
                    "fw_iterations": r["iterations"], "relgap": r["relgap"], "eps": eps},
         "solve_to_1e-6_s": round(solve_s, 3),
         "warm_resolve_s": round(resolve, 4),
@@ -274,6 +285,8 @@ def main():
     ap.add_argument("--workload", default="metro",
                     choices=["grid5", "metro", "regional", "batched", "gradient"])
     ap.add_argument("--variants", type=int, default=512, help="batched: number of variants")
+    ap.add_argument("--grad-network", default="metro", choices=["grid5", "metro", "regional"],
+                    help="gradient: network solved to relgap 1e-6")
     ap.add_argument("--plain-sample", type=int, default=8,
                     help="batched: variants solved one by one for the plain comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
