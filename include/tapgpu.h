@@ -197,6 +197,8 @@ typedef struct {
   int64_t origin_chase;         /* chained expansions per thread and round   */
   double origin_delta;          /* bucket width of the labels-in-HBM kernel, */
                                 /*   in mean free-flow link times (0: none)  */
+  int64_t origin_split;         /* origins per variant co-run by the HBM     */
+                                /*   kernel beside the LDS kernel            */
 } tapgpu_stats;
 
 /* Counters and wall times of the last tapgpu_fw_run (or of the calls since
@@ -230,6 +232,15 @@ int tapgpu_get_origin_chase(const tapgpu_ctx *ctx);
    on it. */
 int tapgpu_set_origin_delta(tapgpu_ctx *ctx, double mult);
 double tapgpu_get_origin_delta(const tapgpu_ctx *ctx);
+
+/* Co-scheduling on the LDS path (strategy 2): the last `origins` origins of
+   every variant run in the labels-in-HBM kernel on a second queue, beside
+   the LDS kernel on the same CUs (whose workgroups leave half of each CU's
+   wave slots idle).  0 (the default): off; -1: tuned on the next loading
+   over 0, 10, 15, 20, 25, 33 percent of the zones.  Results do not depend
+   on it.  (On metro the best split gains about 2.5%.) */
+int tapgpu_set_origin_split(tapgpu_ctx *ctx, int origins);
+int tapgpu_get_origin_split(const tapgpu_ctx *ctx);
 int tapgpu_get_strategy(const tapgpu_ctx *ctx);
 
 /* --- equilibrium gradients ------------------------------------------------ */

@@ -234,6 +234,11 @@ def test_schedule_knobs_do_not_change_results(engine_mod):
             out.append(eng.aon())
         with pytest.raises(engine_mod.TapGpuError):
             eng.set_origin_chase(-2)
+        eng.set_origin_chase(0)
+        for sp in (50, 200):                  # co-run share for the HBM kernel
+            eng.set_origin_split(sp)
+            out.append(eng.aon())
+        eng.set_origin_split(0)
         eng.set_strategy("hbm")
         for d in (0.0, 0.5, 4.0, 64.0):
             eng.set_origin_delta(d)

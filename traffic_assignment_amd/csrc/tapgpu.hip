@@ -437,6 +437,7 @@ struct OriginArgs {
   const unsigned char* var_on;       // [V] or null
   int chase;                         // expansions a thread chains per round
   double delta;                      // labels-in-HBM kernel: bucket width (<= 0: none)
+  int z0, zn;                        // labels-in-HBM kernel: origins [z0, z0 + zn) of each variant
   double* y;                         // [V * E], zeroed by the caller
   int* err;                          // [3]
   unsigned long long* counters;      // [16] expansions, links scanned, updates,
@@ -747,8 +748,8 @@ __global__ __launch_bounds__(kHbmBlock) void k_origin_hbm(OriginArgs A, int load
   unsigned long long c_relax = 0, c_pred = 0, c_load = 0;
 
   for (int item = blockIdx.x; item < items; item += gridDim.x) {
-    const int var = A.src_list ? 0 : item / A.Z;
-    const int src = A.src_list ? A.src_list[item] : item - var * A.Z;
+    const int var = A.src_list ? 0 : item / A.zn;
+    const int src = A.src_list ? A.src_list[item] : A.z0 + item - var * A.zn;
     if (A.var_on && !A.var_on[var]) continue;
     const double* __restrict__ fw = A.fww + (size_t)var * A.E;
     const double* __restrict__ rw = A.rvw + (size_t)var * A.E;
@@ -1174,6 +1175,10 @@ struct tapgpu_ctx {
   // strategy: 0 auto, 1 node-major sweeps, 2 one origin per workgroup (LDS)
   int strategy = 0;
   int chase = -1;                     // LDS path: chained expansions (-1: tune)
+  int split = 0;                      // LDS path: origins per variant run by the HBM kernel
+                                      // alongside it (-1: tune, 0: none)
+  hipStream_t stream2 = nullptr;      // second queue for that co-running kernel
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool hbm_ok = false;                // labels-in-HBM path: bitmaps fit LDS
   size_t hbm_lds = 0;
   int hbm_grid = 0;                   // persistent workgroups
@@ -1452,6 +1457,8 @@ OriginArgs make_origin_args(tapgpu_ctx* c, const unsigned char* var_on) {
   A.chase = c->chase >= 0 ? c->chase : kDefaultChase;
   A.h_dist = c->h_dist; A.h_rnd = c->h_rnd; A.h_pred = c->h_pred; A.h_ptail = c->h_ptail;
   A.delta = c->delta_mult > 0 ? c->delta_mult * c->mean_t0 : 0.0;
+  A.z0 = 0;
+  A.zn = c->Z;
   return A;
 }
 
@@ -1464,21 +1471,78 @@ OriginArgs make_origin_args(tapgpu_ctx* c, const unsigned char* var_on) {
 // below, with the chosen value, is the one whose volumes are kept).
 constexpr int kChaseCandidates[] = {0, 2, 4, 8, 16, 32};
 
+int ensure_hbm(tapgpu_ctx* c);
+
+// One loading launch of the LDS path.  With split > 0 the last `split`
+// origins of every variant go to the labels-in-HBM kernel on a second queue,
+// running on the same CUs at the same time: the LDS kernel's workgroup takes
+// most of a CU's LDS but only half of its wave slots, and the HBM kernel's
+// workgroups (a few KB of LDS each) fill the rest.  Both add integer flows
+// to y with atomics, so the split changes no result.
 int launch_origin(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on, float* ms) {
   HIPCHK(c, hipMemsetAsync(c->y + (size_t)v0 * c->E, 0, sizeof(double) * (size_t)Vn * c->E, c->stream));
   if (c->Z > 0) {
     OriginArgs A = make_origin_args(c, var_on);
     A.fww += (size_t)v0 * c->E; A.rvw += (size_t)v0 * c->E; A.y += (size_t)v0 * c->E;
     if (var_on) A.var_on = var_on + v0;
+    const int zh = (c->hbm_ok && c->split > 0) ? std::min(c->split, c->Z - 1) : 0;
+    const int zl = c->Z - zh;
+    if (zh > 0) {
+      int rc = ensure_hbm(c);
+      if (rc) return rc;
+      A.h_dist = c->h_dist; A.h_rnd = c->h_rnd; A.h_pred = c->h_pred; A.h_ptail = c->h_ptail;
+    }
     if (ms) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    hipLaunchKernelGGL(k_origin_lds, dim3(c->Z, Vn), dim3(kLdsBlock), c->lds_bytes, c->stream, A, 1);
-    if (ms) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    if (zh > 0) {
+      HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+      HIPCHK(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+      OriginArgs B = A;
+      B.z0 = zl;
+      B.zn = zh;
+      const int items = zh * Vn;
+      hipLaunchKernelGGL(k_origin_hbm, dim3(std::min(c->hbm_grid, items)), dim3(kHbmBlock),
+                         c->hbm_lds, c->stream2, B, 1, items);
+      HIPCHK(c, hipGetLastError());
+      HIPCHK(c, hipEventRecord(c->ev_join, c->stream2));
+    }
+    hipLaunchKernelGGL(k_origin_lds, dim3(zl, Vn), dim3(kLdsBlock), c->lds_bytes, c->stream, A, 1);
     HIPCHK(c, hipGetLastError());
+    if (zh > 0) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    if (ms) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     if (ms) {
       HIPCHK(c, hipEventSynchronize(c->ev[1]));
       HIPCHK(c, hipEventElapsedTime(ms, c->ev[0], c->ev[1]));
     }
   }
+  return TAPGPU_OK;
+}
+
+// Share of origins for the co-running HBM kernel: each candidate (as a
+// fraction of Z) is timed on the full launch; the fastest is kept.
+constexpr int kSplitCandidates[] = {0, 10, 15, 20, 25, 33};   // percent of Z
+
+int tune_split(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
+  if (!c->hbm_ok || c->Z < 8) {
+    c->split = 0;
+    return TAPGPU_OK;
+  }
+  if (!c->ev[0])
+    for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
+  float best_ms = 0.f;
+  int best = 0;
+  for (int pct : kSplitCandidates) {
+    c->split = (int)((long long)c->Z * pct / 100);
+    float ms = 0.f;
+    int rc = launch_origin(c, v0, Vn, var_on, &ms);
+    if (rc) return rc;
+    int herr = 0;
+    HIPCHK(c, hipMemcpyAsync(&herr, c->err_dev, sizeof herr, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (herr) break;
+    if (best_ms == 0.f || ms < best_ms) { best_ms = ms; best = c->split; }
+  }
+  c->split = best;
+  c->stats.origin_tune_ms += best_ms;
   return TAPGPU_OK;
 }
 
@@ -1530,6 +1594,7 @@ int finish_origin(tapgpu_ctx* c, int Vn, std::chrono::steady_clock::time_point t
   }
   c->stats.origin_blocks += (int64_t)c->Z * Vn;
   c->stats.origin_chase = c->chase;
+  c->stats.origin_split = c->split;
   c->stats.aon_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (herr[0] == 1)
     return c->fail(TAPGPU_E_NOPATH, "no path from %d to %d", herr[1] + 1, herr[2] + 1);
@@ -1545,7 +1610,15 @@ int aon_lds(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
   auto t0 = std::chrono::steady_clock::now();
   HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
   if (c->chase < 0 && c->Z > 0) {
+    const int keep = c->split;
+    c->split = 0;                       // chain length first, on the LDS kernel alone
     int rc = tune_chase(c, v0, Vn, var_on);
+    c->split = keep;
+    if (rc) return rc;
+    HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
+  }
+  if (c->split < 0 && c->Z > 0) {
+    int rc = tune_split(c, v0, Vn, var_on);
     if (rc) return rc;
     HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
   }
@@ -1874,6 +1947,9 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
   // --- device ---
   HIPCHK(c, hipGetDevice(&c->device));
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+  HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+  HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
   HIPCHK(c, hipHostMalloc((void**)&c->h_int, 64 * sizeof(int), hipHostMallocDefault));
   int rc;
 #define ALLOC(ptr, n) if ((rc = dalloc(c, &c->ptr, (n)))) return rc
@@ -1969,6 +2045,9 @@ void tapgpu_destroy(tapgpu_ctx* c) {
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   delete c;
 }
 
@@ -2362,6 +2441,14 @@ int tapgpu_set_origin_delta(tapgpu_ctx* c, double mult) {
 }
 
 double tapgpu_get_origin_delta(const tapgpu_ctx* c) { return c ? c->delta_mult : -2.0; }
+
+int tapgpu_set_origin_split(tapgpu_ctx* c, int origins) {
+  if (!c || origins < -1) return TAPGPU_E_ARG;
+  c->split = origins;
+  return TAPGPU_OK;
+}
+
+int tapgpu_get_origin_split(const tapgpu_ctx* c) { return c ? c->split : TAPGPU_E_ARG; }
 
 int tapgpu_get_origin_chase(const tapgpu_ctx* c) {
   if (!c) return TAPGPU_E_ARG;

@@ -69,7 +69,8 @@ class _Stats(C.Structure):
                 ("origin_loaded", C.c_int64), ("origin_walk_steps", C.c_int64),
                 ("origin_relax_ms", C.c_double), ("origin_pred_ms", C.c_double),
                 ("origin_load_ms", C.c_double), ("origin_tune_ms", C.c_double),
-                ("origin_chase", C.c_int64), ("origin_delta", C.c_double)]
+                ("origin_chase", C.c_int64), ("origin_delta", C.c_double),
+                ("origin_split", C.c_int64)]
 
 
 class _GradInfo(C.Structure):
@@ -112,6 +113,8 @@ def lib():
         L.tapgpu_set_origin_chase.argtypes = [vp, C.c_int]
         L.tapgpu_get_origin_chase.argtypes = [vp]
         L.tapgpu_set_origin_delta.argtypes = [vp, C.c_double]
+        L.tapgpu_set_origin_split.argtypes = [vp, C.c_int]
+        L.tapgpu_get_origin_split.argtypes = [vp]
         L.tapgpu_get_origin_delta.argtypes = [vp]
         L.tapgpu_get_origin_delta.restype = C.c_double
         L.tapgpu_gradient.argtypes = [vp, C.c_int64, C.c_double, C.c_double, C.c_int,
@@ -277,6 +280,15 @@ class TapEngine:
     @property
     def origin_chase(self):
         return lib().tapgpu_get_origin_chase(self._h)
+
+    def set_origin_split(self, origins):
+        """Origins per variant co-run by the labels-in-HBM kernel beside the
+        LDS kernel (0: none, -1: tune on the next loading)."""
+        self._check(lib().tapgpu_set_origin_split(self._h, int(origins)))
+
+    @property
+    def origin_split(self):
+        return lib().tapgpu_get_origin_split(self._h)
 
     def set_origin_delta(self, mult):
         """Bucket width of the labels-in-HBM path in mean free-flow link
