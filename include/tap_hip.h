@@ -138,6 +138,14 @@ void link_volume_data_setup(link_data_t *links, long num_links, long num_start_n
                             demand_data_t *demands[]);
 void link_volume_data_cleanup(void);
 
+/* The reference's greedy assignment program as a library call: the body of
+   tap_greedy's main (trunk/tap/tap_greedy.c:296-335) after parsing, with the
+   same stderr lines ("initial unsatifised demand", "iter = ...", the
+   unchanged-demand warning, "total cost = ...") and print_flow on flow_fp.
+   The reference has no library function for it.  Returns 0, or 1 on error. */
+int tap_greedy_assign(net_data_t *net, demand_data_t **demands, FILE *flow_fp,
+                      double *total_cost_at_end, int *iterations_at_end);
+
 /* global of the CUDA driver (trunk/tap/tap_frankwolfe_cuda.h:23), kept for
    link compatibility; ignored */
 extern int using_fermi_architecture;

@@ -243,6 +243,31 @@ int tapgpu_set_origin_split(tapgpu_ctx *ctx, int origins);
 int tapgpu_get_origin_split(const tapgpu_ctx *ctx);
 int tapgpu_get_strategy(const tapgpu_ctx *ctx);
 
+/* --- greedy step-cost assignment ------------------------------------------ */
+
+typedef struct {
+  int32_t iterations;       /* assignment iterations done                  */
+  int32_t stalled;          /* stopped because unsatisfied demand changed   */
+                            /*   by less than 1e-8                          */
+  double unsatisfied;       /* demand left unassigned                       */
+  double total_cost;        /* sum x * c at the end                         */
+} tapgpu_greedy_result;
+
+/* The reference's greedy assignment program (trunk/tap/tap_greedy.c:84-150
+   greedy_assign, :300-330 its loop) on variant v, from zero volumes and
+   free-flow costs: per iteration one all-origin shortest-path launch, then
+   the OD pairs in input order each add the flow that brings the first link
+   of their path to the next cost step (trunk/tap/tap_functions.c:241-266),
+   then BPR costs.  Stops when no demand is left, when the unsatisfied total
+   changes by less than 1e-8, or after max_iters (<= 0: no limit).
+   volumes / costs [num_links] (may be NULL) receive the result; the
+   variant's costs are left at the final costs (tapgpu_fw_init resets the FW
+   state).  unsat_trace / ms_trace [trace_cap]: unsatisfied demand and wall
+   time of each iteration. */
+int tapgpu_greedy(tapgpu_ctx *ctx, int64_t variant, int max_iters, double *volumes,
+                  double *costs, double *unsat_trace, double *ms_trace, int trace_cap,
+                  tapgpu_greedy_result *result);
+
 /* --- equilibrium gradients ------------------------------------------------ */
 
 typedef struct {

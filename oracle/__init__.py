@@ -204,5 +204,30 @@ def fw_solve(net, dem, max_iters=0, target_relgap=0.0, nthreads=1, x_init=None):
                 vht=vht.value, trace=tr)
 
 
+def greedy(net, dem, max_iters=0, trace_cap=100000):
+    """Greedy step-cost assignment (tap_greedy.c): dict with volumes x,
+    costs c, iterations, unsatisfied demand (final and per iteration),
+    total cost and whether it stopped on the unchanged-demand test."""
+    L = lib()
+    if not getattr(L, "_greedy_ready", False):
+        L.ora_greedy.argtypes = [C.POINTER(_Net), C.POINTER(_Dem), C.c_int, f64p, f64p,
+                                 C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_void_p, C.c_int]
+        L._greedy_ready = True
+    k = _Keep(net, dem)
+    E = max(net.num_links, 1)
+    x = np.zeros(E); c = np.zeros(E)
+    tr = np.zeros(trace_cap)
+    it, st = C.c_int(0), C.c_int(0)
+    un, tot = C.c_double(0), C.c_double(0)
+    rc = L.ora_greedy(C.byref(k.net), C.byref(k.dem), int(max_iters), x, c, C.byref(it),
+                      C.byref(un), C.byref(tot), C.byref(st), tr.ctypes.data, trace_cap)
+    if rc:
+        raise RuntimeError(f"oracle greedy failed ({rc})")
+    n = it.value
+    return dict(x=x[:net.num_links], c=c[:net.num_links], iterations=n, unsatisfied=un.value,
+                total_cost=tot.value, stalled=bool(st.value), unsat_trace=tr[:min(n, trace_cap)])
+
+
 def ref_available() -> bool:
     return os.path.exists(os.path.join(REF_DIR, "tap_frankwolfe_pthread"))

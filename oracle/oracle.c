@@ -515,3 +515,116 @@ done:
   free(cap); free(Va); free(Ea); free(c); free(x); free(y);
   return rc;
 }
+
+
+/* ------------------------------------------------------------------------ */
+/* greedy step-cost assignment                                               */
+/* ------------------------------------------------------------------------ */
+
+/* distance_from_next_step, tap_functions.c:241-266 (NUM_STEPS = 10,
+   tap_functions.h:24): volume left to the next step of the cost step
+   function; note the reference returns vol itself whenever the next step
+   is at or above vol, which ceil() makes the usual case. */
+static double ora_next_step(double cap, double vol)
+{
+  const double flowstepsize = cap / (double)10;
+  if (vol <= 0.0)
+    return flowstepsize;
+  if (vol >= cap)
+    return DBL_MAX;
+  const double next_step = ceil(vol / flowstepsize);
+  const double next_step_vol = next_step * flowstepsize;
+  if (next_step_vol >= vol)
+    return vol;
+  return next_step_vol - vol;
+}
+
+/* tap_greedy.c: greedy_assign (:84-150) inside the main loop (:300-330):
+   per iteration, for each origin a plain d'Esopo-Pape tree at the current
+   costs; for each of its OD pairs in file order the flow that brings the
+   first link of the path to its next step (at most the remaining demand)
+   is added along the path; then BPR costs; stop when no demand is left or
+   the unsatisfied total changed by less than EPS = 1e-8
+   (tap_functions.h:23).  max_iters <= 0: no limit.  Returns 0, or -2 on
+   bad input; *stalled = 1 when it stopped on the EPS test. */
+int ora_greedy(const ora_net *net, const ora_demand *dem, int max_iters, double *x, double *c,
+               int *iters_out, double *unsat_out, double *total_cost_out, int *stalled,
+               double *unsat_trace, int trace_cap)
+{
+  const int64_t E = net->num_links, N1 = net->num_nodes + 1, Z = net->num_zones;
+  if (E <= 0 || dem->num_zones > Z)
+    return -2;
+  double *cap = malloc(sizeof(double) * E);
+  int64_t *Va = malloc(sizeof(int64_t) * (N1 + 1)), *Ea = malloc(sizeof(int64_t) * E);
+  double *Wa = malloc(sizeof(double) * E);
+  int64_t *prev = malloc(sizeof(int64_t) * N1), *nxt = malloc(sizeof(int64_t) * N1);
+  double *dist = malloc(sizeof(double) * N1);
+  const int64_t nitems = dem->start[dem->num_zones];
+  double *rem = malloc(sizeof(double) * (nitems + 1));
+  ora_build_csr(net, Va, Ea, Wa);
+  /* capacity <= 0 -> smallest positive capacity (tap_greedy.c:270-285) */
+  double min_cap = DBL_MAX;
+  for (int64_t k = 0; k < E; k++)
+    if (net->capacity[k] > 0.0 && net->capacity[k] < min_cap)
+      min_cap = net->capacity[k];
+  for (int64_t k = 0; k < E; k++)
+    cap[k] = net->capacity[k] > 0.0 ? net->capacity[k] : min_cap;
+  for (int64_t k = 0; k < E; k++) {
+    x[k] = 0.0;
+    c[k] = Wa[k];                     /* link_costs = free-flow times */
+  }
+  for (int64_t o = 1; o <= dem->num_zones; o++)
+    for (int64_t i = dem->start[o - 1]; i < dem->start[o]; i++)
+      rem[i] = (dem->dest[i] == o) ? 0.0 : dem->demand[i];
+  double unsat = 0.0;
+  for (int64_t i = 0; i < nitems; i++)
+    unsat += rem[i];
+  int iter = 0;
+  *stalled = 0;
+  while (unsat > 0 && (max_iters <= 0 || iter < max_iters)) {
+    for (int64_t o = 1; o <= dem->num_zones; o++) {
+      ora_sssp_pape(Va, Ea, c, N1, o, net->first_thru_node, prev, dist, nxt);
+      for (int64_t i = dem->start[o - 1]; i < dem->start[o]; i++) {
+        const int64_t d = dem->dest[i];
+        const double routeflow = rem[i];
+        if (o == d || routeflow <= 0)
+          continue;
+        double addflow = routeflow;
+        int fail = 0;
+        for (int64_t v = d; v != o;) {
+          const int64_t k = prev[v];
+          if (k < 0) { fail = 1; break; }
+          const double fns = ora_next_step(cap[k], x[k]);
+          if (fns < addflow)
+            addflow = fns;
+          v = net->init[k];
+        }
+        if (fail)
+          continue;
+        for (int64_t v = d; v != o;) {
+          const int64_t k = prev[v];
+          x[k] += addflow;
+          v = net->init[k];
+        }
+        rem[i] -= addflow;
+      }
+    }
+    ora_update_costs(net, cap, x, c);
+    const double prev_unsat = unsat;
+    unsat = 0.0;
+    for (int64_t i = 0; i < nitems; i++)
+      unsat += rem[i];
+    if (unsat_trace && iter < trace_cap)
+      unsat_trace[iter] = unsat;
+    iter++;
+    if (fabs(unsat - prev_unsat) < 1e-08) {
+      *stalled = 1;
+      break;
+    }
+  }
+  *iters_out = iter;
+  *unsat_out = unsat;
+  *total_cost_out = ora_total_cost(E, x, c);
+  free(cap); free(Va); free(Ea); free(Wa); free(prev); free(nxt); free(dist); free(rem);
+  return 0;
+}

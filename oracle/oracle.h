@@ -95,6 +95,13 @@ int ora_fw_solve(const ora_net *net, const ora_demand *dem, int max_iters,
                  double *x_out, double *c_out, ora_iter *trace, int trace_cap,
                  int *iters_out, double *relgap_out, double *obj_out, double *vht_out);
 
+/* Greedy step-cost assignment (tap_greedy.c:84-150, :300-330).  x, c:
+   [E] volumes and costs at the end; unsat_trace (may be NULL): unsatisfied
+   demand after each iteration. */
+int ora_greedy(const ora_net *net, const ora_demand *dem, int max_iters, double *x, double *c,
+               int *iters_out, double *unsat_out, double *total_cost_out, int *stalled,
+               double *unsat_trace, int trace_cap);
+
 /* Path-based equilibrium (equilibrium.c): gradient projection with column
    generation, for the finite-difference pins of the gradients.  The handle
    is opaque; see equilibrium.c for the method. */

@@ -11,7 +11,9 @@ What it records
   * per-iteration Frank-Wolfe scalars (lambda, gap, objective, lower bound,
     best lower bound, relative gap, average excess cost, VHT) and final link
     volumes/costs from the reference's functions, after checking that loop
-    against the reference's own tap_frankwolfe_pthread binary (-n 1 -v).
+    against the reference's own tap_frankwolfe_pthread binary (-n 1 -v);
+  * the greedy step-cost assignment of the reference's tap_greedy binary
+    (printed unsatisfied demand per iteration, total cost, flows).
 
 Doubles are stored bit-exactly (hex floats or base64 of little-endian bytes).
 Usage: python tests/golden/make_golden.py
@@ -142,6 +144,28 @@ def fw_fixture(name, net, dem, iters, full_arrays, origins_for_sssp):
     print(f"{name}: {trace.shape[0]} iterations, relgap {trace[-1, 5]:.3e}")
 
 
+def greedy_fixture(name, net, dem):
+    """The reference's tap_greedy binary (trunk/tap/tap_greedy.c, built from
+    source): per-iteration unsatisfied demand (%f), total cost (%f) and the
+    final flows table (%.15f)."""
+    with tempfile.TemporaryDirectory() as tmp:
+        nf, tf = write_inputs(tmp, net, dem, name)
+        flows = os.path.join(tmp, "flows.txt")
+        with open(flows, "w") as fo:
+            err = subprocess.run([os.path.join(REF, "tap_greedy"), nf, tf], stdout=fo,
+                                 stderr=subprocess.PIPE, text=True, check=True).stderr
+        _, _, v, c, _ = tntp.read_flows(flows)
+    rows = [l for l in err.splitlines() if l.startswith("iter =")]
+    unsat = [re.search(r"unsatisfied demand = ([-0-9.]+)", l).group(1) for l in rows]
+    total = re.search(r"total cost = ([-0-9.eE+]+)", err).group(1)
+    fix = dict(name=name, inputs_sha256=input_digest(net, dem), iterations=len(rows),
+               unsat_printed=unsat, total_cost_printed=total, x_b64=b64(v), c_b64=b64(c),
+               stalled="unchanged" in err)
+    with open(os.path.join(OUT, f"{name}_greedy.json"), "w") as f:
+        json.dump(fix, f, indent=0)
+    print(f"{name} greedy: {len(rows)} iterations, total cost {total}")
+
+
 def rmat_fixtures():
     files = {"martin": "cuda_sssp/rmat.txt.martin",
              "okuyama": "cuda_sssp/rmat.txt.okuyama-example",
@@ -194,6 +218,8 @@ def main():
     net, dem = synth.metro()
     fw_fixture("metro", net, dem, 3, False, [1, 500, 1000])
     rmat_fixtures()
+    for name, (net, dem) in (("grid5", synth.grid5()), ("metro_s03", synth.metro(scale=0.3))):
+        greedy_fixture(name, net, dem)
 
 
 if __name__ == "__main__":

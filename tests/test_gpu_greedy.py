@@ -1,0 +1,73 @@
+"""Greedy step-cost assignment (tapgpu_greedy, tap_greedy_hip) against the
+oracle restatement of trunk/tap/tap_greedy.c and the reference binary's
+golden output (tests/golden/*_greedy.json)."""
+from __future__ import annotations
+
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import b64_array, golden, network
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CASES = [("grid5", 1.0, "grid5_greedy.json"), ("metro", 0.3, "metro_s03_greedy.json")]
+
+
+@pytest.mark.parametrize("strategy", ["sweeps", "lds", "hbm"])
+@pytest.mark.parametrize("name,scale,fix", CASES, ids=["grid5", "metro0.3"])
+def test_greedy_matches_oracle_and_reference(engine_mod, oracle_lib, name, scale, fix, strategy):
+    print(f"case greedy {name} {strategy}", flush=True)
+    net, dem = network(name, scale)
+    with engine_mod.TapEngine(net, dem) as eng:
+        if strategy == "lds" and eng.strategy != "lds":
+            pytest.skip("network does not fit the LDS path")
+        eng.set_strategy(strategy)
+        g = eng.greedy()
+    # oracle with the engine's correctly rounded x^4: bit for bit
+    oracle_lib.lib().ora_set_pow_mode(1)
+    try:
+        o = oracle_lib.greedy(net, dem)
+    finally:
+        oracle_lib.lib().ora_set_pow_mode(0)
+    assert g["iterations"] == o["iterations"] and g["stalled"] == o["stalled"]
+    assert np.array_equal(g["unsat_trace"], o["unsat_trace"])
+    assert np.array_equal(g["x"], o["x"]) and np.array_equal(g["c"], o["c"])
+    assert g["total_cost"] == o["total_cost"] and g["unsatisfied"] == 0.0
+    # the reference binary (glibc pow): printed values
+    ref = golden(fix)
+    assert g["iterations"] == ref["iterations"]
+    assert [f"{u:.6f}" for u in g["unsat_trace"]] == ref["unsat_printed"]
+    x = b64_array(ref["x_b64"], np.float64)
+    c = b64_array(ref["c_b64"], np.float64)
+    np.testing.assert_allclose(g["x"], x, rtol=0, atol=1e-15 * np.abs(x).max())
+    np.testing.assert_allclose(g["c"], c, rtol=1e-14)
+
+
+def test_greedy_cli_matches_reference_output(engine_mod):
+    """tap_greedy_hip prints what the reference tap_greedy prints."""
+    from traffic_assignment_amd import tntp
+    exe = os.path.join(ROOT, "traffic_assignment_amd", "tap_greedy_hip")
+    if not os.path.exists(exe):
+        pytest.skip("tap_greedy_hip not built")
+    net, dem = network("metro", 0.3)
+    ref = golden("metro_s03_greedy.json")
+    with tempfile.TemporaryDirectory() as td:
+        nf, tf, ff = (os.path.join(td, f) for f in ("net.txt", "trips.txt", "flows.txt"))
+        tntp.write_net(nf, net)
+        tntp.write_trips(tf, dem)
+        with open(ff, "w") as fo:
+            err = subprocess.run([exe, nf, tf], stdout=fo, stderr=subprocess.PIPE, text=True,
+                                 check=True, timeout=120).stderr
+        _, _, v, c, _ = tntp.read_flows(ff)
+    lines = [l for l in err.splitlines() if l.startswith("iter =")]
+    assert len(lines) == ref["iterations"]
+    assert [l.split("unsatisfied demand = ")[1].split()[0] for l in lines] == ref["unsat_printed"]
+    assert f"total cost = {ref['total_cost_printed']}" in err
+    assert "initial unsatifised demand = " in err
+    np.testing.assert_allclose(v, b64_array(ref["x_b64"], np.float64), rtol=0, atol=1e-9)
+    np.testing.assert_allclose(c, b64_array(ref["c_b64"], np.float64), rtol=1e-14)

@@ -67,3 +67,25 @@ def test_sssp_reference_graphs(oracle_lib, tag):
         d, p = oracle_lib.sssp(net, net.fftime, int(o), lll=(alg == "pape_lll"))
         assert sha(d[1:]) == ent["dist_sha256"], key
         assert sha(p[1:]) == ent["pred_sha256"], key
+
+
+GREEDY = [("grid5", 1.0, "grid5_greedy.json"), ("metro", 0.3, "metro_s03_greedy.json")]
+
+
+@pytest.mark.parametrize("name,scale,fix", GREEDY, ids=["grid5", "metro0.3"])
+def test_greedy_matches_reference_binary(oracle_lib, name, scale, fix):
+    """The oracle's greedy step-cost assignment against the reference's
+    tap_greedy binary: same iterations, the same unsatisfied demand after
+    each (as printed, %f), flows equal to the printed %.15f."""
+    g = golden(fix)
+    net, dem = network(name, scale)
+    assert _digest(net, dem) == g["inputs_sha256"]
+    o = oracle_lib.greedy(net, dem)
+    assert o["iterations"] == g["iterations"] and o["stalled"] == g["stalled"]
+    assert [f"{u:.6f}" for u in o["unsat_trace"]] == g["unsat_printed"]
+    assert f"{o['total_cost']:.6f}" == g["total_cost_printed"]
+    x = b64_array(g["x_b64"], np.float64)
+    c = b64_array(g["c_b64"], np.float64)
+    np.testing.assert_allclose(o["x"], x, rtol=0, atol=1e-15 * np.abs(x).max())
+    np.testing.assert_allclose(o["c"], c, rtol=0, atol=5e-16 * np.abs(c).max() + 1e-15)
+    assert o["unsatisfied"] == 0.0

@@ -53,11 +53,13 @@ def build_host():
     lib = os.path.join(HERE, "libtap_hip.so")
     _run(["gcc", *C_FLAGS, "-shared", "-I", INC, "-o", lib, *srcs,
           "-L", HERE, "-ltapgpu", "-Wl,-rpath,$ORIGIN", "-lm"])
-    main = os.path.join(CSRC, "tap_main_hip.c")
-    if os.path.exists(main):
-        exe = os.path.join(HERE, "tap_frankwolfe_hip")
-        _run(["gcc", *C_FLAGS, "-I", INC, "-o", exe, main, "-L", HERE, "-ltap_hip", "-ltapgpu",
-              "-Wl,-rpath,$ORIGIN", "-lm"])
+    for src, name in (("tap_main_hip.c", "tap_frankwolfe_hip"),
+                      ("tap_greedy_main.c", "tap_greedy_hip")):
+        main = os.path.join(CSRC, src)
+        if os.path.exists(main):
+            exe = os.path.join(HERE, name)
+            _run(["gcc", *C_FLAGS, "-I", INC, "-o", exe, main, "-L", HERE, "-ltap_hip",
+                  "-ltapgpu", "-Wl,-rpath,$ORIGIN", "-lm"])
     return lib
 
 

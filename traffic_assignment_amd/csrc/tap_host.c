@@ -494,3 +494,55 @@ done:
   }
   return ret;
 }
+
+
+/* tap_greedy's main loop and output (trunk/tap/tap_greedy.c:296-335) on the
+   engine's greedy assignment (tapgpu_greedy). */
+int tap_greedy_assign(net_data_t *net, demand_data_t **demands, FILE *flow_fp,
+                      double *total_cost_at_end, int *iterations_at_end)
+{
+  const long E = net->num_links;
+  engine_input in;
+  if (make_input(net, demands, &in)) {
+    fprintf(stderr, "malloc failed in tap_greedy_assign\n");
+    return 1;
+  }
+  tapgpu_ctx *ctx = NULL;
+  int rc = tapgpu_create(&ctx, &in.net, &in.dem);
+  if (rc) {
+    fprintf(stderr, "ERROR: %s\n", tapgpu_last_error(NULL));
+    free_input(&in);
+    return 1;
+  }
+  double initial = 0;
+  for (long o = 1; o < net->num_zones + 1; o++)
+    for (long i = 0; demands[o][i].dest != 0; i++)
+      if (demands[o][i].dest != o)
+        initial += demands[o][i].demand;
+  fprintf(stderr, "initial unsatifised demand = %f\n", initial);
+  const int cap = 100000;
+  double *x = malloc(sizeof(double) * (E + 1)), *c = malloc(sizeof(double) * (E + 1));
+  double *un = malloc(sizeof(double) * cap), *ms = malloc(sizeof(double) * cap);
+  tapgpu_greedy_result res;
+  int ret = 0;
+  rc = tapgpu_greedy(ctx, 0, cap, x, c, un, ms, cap, &res);
+  if (rc) {
+    fprintf(stderr, "ERROR: %s\n", tapgpu_last_error(ctx));
+    ret = 1;
+  } else {
+    for (int i = 0; i < res.iterations && i < cap; i++)
+      fprintf(stderr, "iter = %d unsatisfied demand = %f (time %d ms)\n", i + 1, un[i],
+              (int)ms[i]);
+    if (res.stalled)
+      fprintf(stderr, "WARNING: unsatisfied demand unchanged, terminating with unsatisfied "
+                      "demand = %f\n", res.unsatisfied);
+    fprintf(stderr, "total cost = %f\n", res.total_cost);
+    print_flow(flow_fp, net, x, c);
+    *total_cost_at_end = res.total_cost;
+    *iterations_at_end = res.iterations;
+  }
+  tapgpu_destroy(ctx);
+  free_input(&in);
+  free(x); free(c); free(un); free(ms);
+  return ret;
+}

@@ -1190,6 +1190,10 @@ struct tapgpu_ctx {
   int clock_khz = 0;                  // device wall clock
   bool labels_ready = false;          // node-major buffers allocated
   std::vector<double> h_odc;          // demand by (zone row, column), host copy
+  std::vector<long long> h_od_start;  // origin-major demand, host copy (input order)
+  std::vector<int> h_od_dest;
+  std::vector<double> h_od_val;
+  std::vector<int> h_tail;            // tail node of each link (0-based)
   bool lds_ok = false;
   size_t lds_bytes = 0;
   // labels
@@ -1991,6 +1995,11 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
     HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_origin_lds),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_bytes));
 
+  c->h_od_start = std::move(h_od_start);
+  c->h_od_dest = std::move(h_od_dest);
+  c->h_od_val = std::move(h_od_val);
+  c->h_tail = std::move(tail);
+
   HIPCHK(c, hipMemsetAsync(c->x, 0, sizeof(double) * VE, c->stream));
   HIPCHK(c, hipMemsetAsync(c->y, 0, sizeof(double) * VE, c->stream));
   HIPCHK(c, hipMemsetAsync(c->var_on, 1, V, c->stream));
@@ -2472,3 +2481,4 @@ int tapgpu_set_profiling(tapgpu_ctx* c, int on) {
 }  // extern "C"
 
 #include "tapgrad.inc"
+#include "tapgreedy.inc"

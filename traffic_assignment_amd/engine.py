@@ -73,6 +73,11 @@ class _Stats(C.Structure):
                 ("origin_split", C.c_int64)]
 
 
+class _GreedyResult(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("stalled", C.c_int32), ("unsatisfied", C.c_double),
+                ("total_cost", C.c_double)]
+
+
 class _GradInfo(C.Structure):
     _fields_ = [("cycles", C.c_int64), ("nonzeros", C.c_int64), ("useful_rounds", C.c_int32),
                 ("cg_iters", C.c_int32), ("cg_status", C.c_int32), ("reserved", C.c_int32),
@@ -114,6 +119,8 @@ def lib():
         L.tapgpu_get_origin_chase.argtypes = [vp]
         L.tapgpu_set_origin_delta.argtypes = [vp, C.c_double]
         L.tapgpu_set_origin_split.argtypes = [vp, C.c_int]
+        L.tapgpu_greedy.argtypes = [vp, C.c_int64, C.c_int, vp, vp, vp, vp, C.c_int,
+                                    C.POINTER(_GreedyResult)]
         L.tapgpu_get_origin_split.argtypes = [vp]
         L.tapgpu_get_origin_delta.argtypes = [vp]
         L.tapgpu_get_origin_delta.restype = C.c_double
@@ -262,6 +269,20 @@ class TapEngine:
                                           C.byref(info)))
         out["info"] = {f: getattr(info, f) for f, _ in _GradInfo._fields_ if f != "reserved"}
         return out
+
+    def greedy(self, variant=0, max_iters=0, trace_cap=10000):
+        """The reference's greedy step-cost assignment (tapgpu_greedy): dict
+        with volumes x, costs c, iterations, stalled, unsatisfied,
+        total_cost and per-iteration unsat_trace / ms_trace."""
+        x = np.empty(self.E); c = np.empty(self.E)
+        un = np.zeros(trace_cap); ms = np.zeros(trace_cap)
+        res = _GreedyResult()
+        self._check(lib().tapgpu_greedy(self._h, variant, int(max_iters), _p(x), _p(c), _p(un),
+                                        _p(ms), trace_cap, C.byref(res)))
+        n = min(res.iterations, trace_cap)
+        return dict(x=x, c=c, iterations=res.iterations, stalled=bool(res.stalled),
+                    unsatisfied=res.unsatisfied, total_cost=res.total_cost,
+                    unsat_trace=un[:n], ms_trace=ms[:n])
 
     STRATEGIES = {"auto": 0, "sweeps": 1, "lds": 2, "hbm": 3}
 
