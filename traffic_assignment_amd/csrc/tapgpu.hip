@@ -586,18 +586,29 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
       if (v != src && dv != kMax) {
         unsigned best = 0xFFFFFFFFu;
         unsigned long long best_du = ~0ull;
-        for (int k = A.rv_off[v]; k < A.rv_off[v + 1]; ++k) {
-          const int u = A.rv_src[k];
-          if (u == v || (u < A.ft0 && u != src)) continue;
-          const unsigned long long du = dist[u];
-          if (du == kMax) continue;
-          if (as_b(as_d(du) + rw[k]) != dv) continue;
-          const unsigned key = ((unsigned)rnd[u] << 16) | (unsigned)u;
-          if (du < best_du || (du == best_du && key < best)) {
-            best = key;
-            best_du = du;
-            p = (unsigned short)A.rv_eid[k];
-            pt = (unsigned short)u;
+        const int kb = A.rv_off[v], ke = A.rv_off[v + 1];
+        // tails and costs of up to 8 in-links are loaded before any is tested
+        for (int k0 = kb; k0 < ke; k0 += 8) {
+          int hu[8];
+          double hw[8];
+#pragma unroll
+          for (int i = 0; i < 8; ++i)
+            if (k0 + i < ke) { hu[i] = A.rv_src[k0 + i]; hw[i] = rw[k0 + i]; }
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            if (k0 + i >= ke) break;
+            const int u = hu[i];
+            if (u == v || (u < A.ft0 && u != src)) continue;
+            const unsigned long long du = dist[u];
+            if (du == kMax) continue;
+            if (as_b(as_d(du) + hw[i]) != dv) continue;
+            const unsigned key = ((unsigned)rnd[u] << 16) | (unsigned)u;
+            if (du < best_du || (du == best_du && key < best)) {
+              best = key;
+              best_du = du;
+              p = (unsigned short)A.rv_eid[k0 + i];
+              pt = (unsigned short)u;
+            }
           }
         }
       }
@@ -852,18 +863,34 @@ __global__ __launch_bounds__(kHbmBlock) void k_origin_hbm(OriginArgs A, int load
       const unsigned long long dv = ldg_u64(&dist[v]);
       if (v != src && dv != kMax) {
         unsigned long long best = ~0ull, best_du = ~0ull;
-        for (int k = A.rv_off[v]; k < A.rv_off[v + 1]; ++k) {
-          const int u = A.rv_src[k];
-          if (u == v || (u < A.ft0 && u != src)) continue;
-          const unsigned long long du = ldg_u64(&dist[u]);
-          if (du == kMax) continue;
-          if (as_b(as_d(du) + rw[k]) != dv) continue;
-          const unsigned long long key = ((unsigned long long)ldg_u32(&rnd[u]) << 32) | (unsigned)u;
-          if (du < best_du || (du == best_du && key < best)) {
-            best = key;
-            best_du = du;
-            p = A.rv_eid[k];
-            pt = u;
+        const int kb = A.rv_off[v], ke = A.rv_off[v + 1];
+        // tails, costs and tail labels of up to 8 in-links are loaded
+        // before any is tested
+        for (int k0 = kb; k0 < ke; k0 += 8) {
+          int hu[8];
+          double hw[8];
+          unsigned long long hd[8];
+#pragma unroll
+          for (int i = 0; i < 8; ++i)
+            if (k0 + i < ke) { hu[i] = A.rv_src[k0 + i]; hw[i] = rw[k0 + i]; }
+#pragma unroll
+          for (int i = 0; i < 8; ++i)
+            if (k0 + i < ke) hd[i] = ldg_u64(&dist[hu[i]]);
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            if (k0 + i >= ke) break;
+            const int u = hu[i];
+            if (u == v || (u < A.ft0 && u != src)) continue;
+            const unsigned long long du = hd[i];
+            if (du == kMax) continue;
+            if (as_b(as_d(du) + hw[i]) != dv) continue;
+            const unsigned long long key = ((unsigned long long)ldg_u32(&rnd[u]) << 32) | (unsigned)u;
+            if (du < best_du || (du == best_du && key < best)) {
+              best = key;
+              best_du = du;
+              p = A.rv_eid[k0 + i];
+              pt = u;
+            }
           }
         }
       }
