@@ -199,6 +199,7 @@ typedef struct {
                                 /*   in mean free-flow link times (0: none)  */
   int64_t origin_split;         /* origins per variant co-run by the HBM     */
                                 /*   kernel beside the LDS kernel            */
+  int64_t origin_deferred;      /* queued nodes kept for a later bucket      */
 } tapgpu_stats;
 
 /* Counters and wall times of the last tapgpu_fw_run (or of the calls since

@@ -756,6 +756,7 @@ __global__ __launch_bounds__(kHbmBlock) void k_origin_hbm(OriginArgs A, int load
   if (tid < 8) s_cnt[tid] = 0;
   if (tid == 0) s_maxlen = 0;
   unsigned long long n_exp = 0, n_scan = 0, n_upd = 0, n_rounds = 0, n_steps = 0, n_loaded = 0;
+  unsigned long long n_def = 0;
   unsigned long long c_relax = 0, c_pred = 0, c_load = 0;
 
   for (int item = blockIdx.x; item < items; item += gridDim.x) {
@@ -801,6 +802,7 @@ __global__ __launch_bounds__(kHbmBlock) void k_origin_hbm(OriginArgs A, int load
             atomicOr(&nxt[u >> 6], 1ull << (u & 63));
             atomicMin(&s_minq[sl], dub);
             s_more[sl] = 1;
+            ++n_def;
             continue;
           }
           s_exp[sl] = 1;
@@ -864,20 +866,21 @@ __global__ __launch_bounds__(kHbmBlock) void k_origin_hbm(OriginArgs A, int load
       if (v != src && dv != kMax) {
         unsigned long long best = ~0ull, best_du = ~0ull;
         const int kb = A.rv_off[v], ke = A.rv_off[v + 1];
-        // tails, costs and tail labels of up to 8 in-links are loaded
-        // before any is tested
-        for (int k0 = kb; k0 < ke; k0 += 8) {
-          int hu[8];
-          double hw[8];
-          unsigned long long hd[8];
+        // tails, costs and tail labels of up to 4 in-links are loaded
+        // before any is tested (4, not 8: registers decide how many
+        // workgroups a CU holds)
+        for (int k0 = kb; k0 < ke; k0 += 4) {
+          int hu[4];
+          double hw[4];
+          unsigned long long hd[4];
 #pragma unroll
-          for (int i = 0; i < 8; ++i)
+          for (int i = 0; i < 4; ++i)
             if (k0 + i < ke) { hu[i] = A.rv_src[k0 + i]; hw[i] = rw[k0 + i]; }
 #pragma unroll
-          for (int i = 0; i < 8; ++i)
+          for (int i = 0; i < 4; ++i)
             if (k0 + i < ke) hd[i] = ldg_u64(&dist[hu[i]]);
 #pragma unroll
-          for (int i = 0; i < 8; ++i) {
+          for (int i = 0; i < 4; ++i) {
             if (k0 + i >= ke) break;
             const int u = hu[i];
             if (u == v || (u < A.ft0 && u != src)) continue;
@@ -952,9 +955,10 @@ __global__ __launch_bounds__(kHbmBlock) void k_origin_hbm(OriginArgs A, int load
   }
   if (A.counters) {
     atomicAdd(&s_cnt[0], n_exp); atomicAdd(&s_cnt[1], n_scan); atomicAdd(&s_cnt[2], n_upd);
-    atomicAdd(&s_cnt[5], n_loaded); atomicAdd(&s_cnt[6], n_steps);
+    atomicAdd(&s_cnt[5], n_loaded); atomicAdd(&s_cnt[6], n_steps); atomicAdd(&s_cnt[7], n_def);
     __syncthreads();
     if (tid == 0) {
+      atomicAdd(&A.counters[7], s_cnt[7]);
       atomicAdd(&A.counters[0], s_cnt[0]); atomicAdd(&A.counters[1], s_cnt[1]);
       atomicAdd(&A.counters[2], s_cnt[2]); atomicAdd(&A.counters[3], n_rounds);
       atomicAdd(&A.counters[4], (unsigned long long)s_maxlen);
@@ -1612,6 +1616,7 @@ int finish_origin(tapgpu_ctx* c, int Vn, std::chrono::steady_clock::time_point t
   c->stats.origin_load_rounds += (int64_t)hc[4];
   c->stats.origin_loaded += (int64_t)hc[5];
   c->stats.origin_walk_steps += (int64_t)hc[6];
+  c->stats.origin_deferred += (int64_t)hc[7];
   if (c->clock_khz > 0) {
     c->stats.origin_relax_ms += (double)hc[8] / c->clock_khz;
     c->stats.origin_pred_ms += (double)hc[9] / c->clock_khz;

@@ -70,7 +70,7 @@ class _Stats(C.Structure):
                 ("origin_relax_ms", C.c_double), ("origin_pred_ms", C.c_double),
                 ("origin_load_ms", C.c_double), ("origin_tune_ms", C.c_double),
                 ("origin_chase", C.c_int64), ("origin_delta", C.c_double),
-                ("origin_split", C.c_int64)]
+                ("origin_split", C.c_int64), ("origin_deferred", C.c_int64)]
 
 
 class _GreedyResult(C.Structure):
