@@ -36,9 +36,9 @@ PEAK_HBM_GBS = 8000.0
 TARGET = 1e-4
 
 DESCR = {
-    "grid5": "grid5 (config 1): 24 nodes / 76 links / 24 zones",
-    "metro": "metro (config 2): 11,000 nodes / 43,600 links / 1,000 zones, gravity demand 400k",
-    "regional": "regional (config 3): 94,000 nodes / 386,800 links / 4,000 zones",
+    "grid5": "grid5: 24 nodes / 76 links / 24 zones",
+    "metro": "metro: 11,000 nodes / 43,600 links / 1,000 zones, gravity demand 400k",
+    "regional": "regional: 94,000 nodes / 386,800 links / 4,000 zones",
 }
 
 
@@ -121,7 +121,7 @@ def load_traffic(workload):
 
 
 def bench_batched(args):
-    """Config 4: --variants upgrade subsets of the metro network solved to
+    """Batched form: --variants upgrade subsets of the metro network solved to
     relgap 1e-4 in one batch (one launch per FW step for all variants),
     against the same engine solving a sample of the variants one at a time."""
     import torch
@@ -187,7 +187,7 @@ def bench_batched(args):
         "steps": args.steps, "warmup": 0, "ms_per_step": round(1e3 * elapsed / args.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded generator, traffic_assignment_amd/synth.py)",
-        "config": {"workload": f"batched (config 4): {len(subsets)} upgrade subsets of metro, "
+        "config": {"workload": f"batched: {len(subsets)} upgrade subsets of metro, "
                                "32 candidate upgrades, budget 40", "target_relgap": TARGET,
                    "engine_path": path, "iterations_min_median_max":
                        [min(iters), int(np.median(iters)), max(iters)]},
@@ -300,7 +300,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
-              "design is described in DESIGN.md", file=sys.stderr)
+              "sharding of this path (by origin or by variant) is not implemented", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -86,7 +86,8 @@ def gpu_available() -> bool:
 
 @pytest.fixture(scope="session")
 def engine_mod():
-    """The product's Python front end; on a GPU box a missing library fails."""
+    """The product's Python front end; where a GPU is present a missing
+    library fails instead of skipping."""
     if not gpu_available():
         pytest.skip("no GPU")
     from traffic_assignment_amd import engine

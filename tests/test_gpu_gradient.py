@@ -55,7 +55,7 @@ def test_gradient_matches_path_reference(engine_mod, name, sc):
 
 
 def test_gradient_matches_finite_differences(engine_mod):
-    """Config-5 check at metro scale 0.3: 64 random loaded links, central
+    """Gradient check at metro scale 0.3: 64 random loaded links, central
     differences of re-solved equilibria.  Step 1e-4 * c: 1e-4 relative above
     the re-solve noise (~1e-7..1e-6 absolute, bounded by 1e-5 of the largest
     gradient in the sample); step 1e-3 * c (ten times less noise, ~4e-8
