@@ -405,8 +405,10 @@ __global__ __launch_bounds__(kBlock) void k_aon_reduce(int E, int Gv, int v0, in
 //
 // For a metro-sized network one origin's labels (8 B/node) fit in a CU's
 // 160 KB LDS, so a 1024-thread workgroup runs the whole origin: rounds of
-// label-correcting relaxation (frontier bitmaps, 64-bit LDS atomicMin on the
-// label bits -- non-negative doubles order like their bit patterns), then
+// label-correcting relaxation (64-bit LDS atomicMin on the label bits --
+// non-negative doubles order like their bit patterns; the nodes lowered in a
+// round form the next round's frontier, deduplicated by a bitmap and listed
+// in an LDS queue so that each frontier node gets a thread of its own), then
 // predecessor selection, then loading: each destination walks its tree path
 // adding its demand to LDS node flows (FP64 LDS atomics), and every tree
 // link with flow adds it to the HBM link volumes (FP64 atomics).  One launch
@@ -424,6 +426,14 @@ constexpr int kLdsBlock = 1024;
 constexpr int kDefaultChase = 4;               // before tuning / shortest-path-only calls
 constexpr int kLdsMaxNodes = 16384;            // per-thread register arrays below
 constexpr int kLdsPerThread = kLdsMaxNodes / kLdsBlock;
+#ifndef TAPGPU_LDS_QUEUE_MAX
+#define TAPGPU_LDS_QUEUE_MAX 4096
+#endif
+constexpr int kLdsQueueMax = TAPGPU_LDS_QUEUE_MAX;   // frontier queue entries per round
+#ifndef TAPGPU_LDS_GROUP
+#define TAPGPU_LDS_GROUP 8
+#endif
+constexpr int kGroup = TAPGPU_LDS_GROUP;             // lanes per expanded node (one link each)
 constexpr unsigned short kNoPred = 0xFFFF;
 
 struct OriginArgs {
@@ -436,6 +446,7 @@ struct OriginArgs {
   const double* od_val;
   const unsigned char* var_on;       // [V] or null
   int chase;                         // expansions a thread chains per round
+  int qcap;                          // LDS frontier queue entries per round (0: bitmap scan only)
   double delta;                      // labels-in-HBM kernel: bucket width (<= 0: none)
   int z0, zn;                        // labels-in-HBM kernel: origins [z0, z0 + zn) of each variant
   double* y;                         // [V * E], zeroed by the caller
@@ -461,11 +472,11 @@ __device__ __forceinline__ double as_d(unsigned long long b) { return __longlong
 __device__ __forceinline__ unsigned long long as_b(double d) { return (unsigned long long)__double_as_longlong(d); }
 
 // LDS layout (bytes): labels 8N (flows while loading) | three bitmaps
-// 3*8W | offs/pred 2A | rnd/ptail 2A, with W = ceil(N/64), A = N+1 rounded
-// up to 4.
-size_t origin_lds_bytes(int N) {
+// 3*8W | offs/pred 2A | rnd/ptail 2A | two frontier queues 2*2Q, with
+// W = ceil(N/64), A = N+1 rounded up to 4, Q = queue entries.
+size_t origin_lds_bytes(int N, int qcap) {
   const size_t W = (size_t)(N + 63) / 64, A = (size_t)((N + 1 + 3) & ~3);
-  return 8ull * N + 24ull * W + 4ull * A;
+  return 8ull * N + 24ull * W + 4ull * A + 4ull * (size_t)qcap;
 }
 
 __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load) {
@@ -481,7 +492,10 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
   unsigned short* pred = offs;
   unsigned short* rnd = offs + Al;                                 // [N], then ptail [N]
   unsigned short* ptail = rnd;
-  __shared__ int s_more[3];
+  unsigned short* queue0 = rnd + Al;                               // 2 x [qcap]
+  unsigned short* queue1 = queue0 + A.qcap;
+  auto qw_of = [&](int rr) { return (rr & 1) ? queue1 : queue0; };   // queue written in round rr
+  __shared__ int s_qn[3];
   __shared__ unsigned long long s_cnt[8];
 
   const int var = blockIdx.y;
@@ -495,78 +509,111 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
   for (int v = tid; v < N; v += kLdsBlock) { dist[v] = kMax; rnd[v] = 0; }
   for (int v = tid; v <= N; v += kLdsBlock) offs[v] = (unsigned short)A.fw_off[v];
   for (int w = tid; w < 3 * W; w += kLdsBlock) bmA[w] = 0;
-  if (tid < 3) s_more[tid] = 0;
+  if (tid < 3) s_qn[tid] = tid == 0 ? 1 : 0;
   if (tid < 8) s_cnt[tid] = 0;
   __syncthreads();
   if (tid == 0) {
     dist[src] = 0ull;                     // +0.0
     bmA[src >> 6] = 1ull << (src & 63);
+    if (A.qcap > 0) queue0[0] = (unsigned short)src;
   }
   __syncthreads();
 
   unsigned long long n_exp = 0, n_scan = 0, n_upd = 0;
   const long long clk0 = wall_clock64();
-  // ---- label-correcting rounds (one barrier each: three rotating bitmaps,
-  //      the third one is cleared while the first is read).  A thread that
-  //      lowers a label expands that node itself, up to A.chase times in a
-  //      row, instead of leaving it to the next round: chains of several
-  //      hops settle per barrier.  Labels are the same least fixed point for
-  //      any such order. ----
+  // ---- label-correcting rounds, one barrier each.  Round r expands the
+  //      nodes lowered in round r-1.  Normally they are listed in a queue
+  //      (count s_qn[(r-1)%3], buffer (r-1)&1) and each gets a group of
+  //      kGroup lanes, one out-link per lane: the group's loads are
+  //      consecutive, and a round costs one link's chain of latencies.
+  //      When more nodes were lowered than the queue holds, the round scans
+  //      their bitmap `cur` in 16-bit slots instead, a lane expanding each
+  //      set bit's node (and chaining up to A.chase further expansions of
+  //      nodes it lowers itself).  A node lowered in round r is listed once:
+  //      the lane whose atomicOr sets its bit in `nxt` appends it to queue
+  //      r&1, one LDS atomicAdd per wave for all of the wave's appends.
+  //      Three rotating bitmaps: the third one is cleared while the first
+  //      is read.  Labels are the same least fixed point for any such
+  //      order. ----
   unsigned long long *cur = bmA, *nxt = bmB, *clr = bmC;
+  const int lane = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
   int r = 1;
+  // v was lowered by this lane (cand); list it if this lane set its bit
+  auto list = [&](int v, bool cand) {
+    bool first = false;
+    if (cand) {
+      const unsigned long long bit = 1ull << (v & 63);
+      first = !(atomicOr(&nxt[v >> 6], bit) & bit);
+    }
+    const unsigned long long bal = __ballot(first);
+    if (bal) {
+      const int leader = __ffsll((long long)bal) - 1;
+      int b0 = 0;
+      if (lane == leader) b0 = atomicAdd(&s_qn[r % 3], __popcll(bal));
+      b0 = __builtin_amdgcn_readlane(b0, leader);
+      if (first) {
+        const int pos = b0 + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+        if (pos < A.qcap) qw_of(r)[pos] = (unsigned short)v;
+      }
+    }
+  };
+  // relaxes u -> v at cost w; true if this lane lowered v
+  auto relax = [&](double du, int v, double w) {
+    const unsigned long long cb = as_b(du + w);
+    if (cb < dist[v] && cb < atomicMin(&dist[v], cb)) {
+      ++n_upd;
+      rnd[v] = (unsigned short)r;
+      return true;
+    }
+    return false;
+  };
   for (;; ++r) {
-    for (int slot = tid; slot < 4 * W; slot += kLdsBlock) {
-      const int w = slot >> 2, q = slot & 3;
-      unsigned bits = (unsigned)((cur[w] >> (16 * q)) & 0xFFFFull);
-      while (bits) {
-        const int b = __ffs(bits) - 1;
-        bits &= bits - 1;
-        int u = (w << 6) + 16 * q + b;
-        for (int hops = 0;; ++hops) {
-          const double du = as_d(dist[u]);
-          const int eb = offs[u], ee = offs[u + 1];
-          ++n_exp;
-          n_scan += (unsigned long long)(ee - eb);
-          int next = -1;
-          // up to 8 links' heads and costs are loaded before any of them is
-          // relaxed: one L2 round trip per expansion instead of one per link
-          for (int e0 = eb; e0 < ee; e0 += 8) {
-            int hv[8];
-            double hw[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-              if (e0 + k < ee) { hv[k] = A.fw_dst[e0 + k]; hw[k] = fw[e0 + k]; }
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-              if (e0 + k >= ee) break;
-              const int v = hv[k];
-              const unsigned long long cb = as_b(du + hw[k]);
-              if (cb < dist[v]) {
-                const unsigned long long old = atomicMin(&dist[v], cb);
-                if (cb < old) {
-                  ++n_upd;
-                  rnd[v] = (unsigned short)r;
-                  if (v >= A.ft0) {
-                    if (next < 0 && hops < A.chase) {
-                      next = v;
-                    } else {
-                      atomicOr(&nxt[v >> 6], 1ull << (v & 63));
-                      s_more[r % 3] = 1;
-                    }
-                  }
-                }
-              }
+    const int qn = s_qn[(r - 1) % 3];
+    if (qn <= A.qcap) {
+      const unsigned short* qr = (r & 1) ? queue0 : queue1;
+      const int g = tid & (kGroup - 1);
+      for (int idx = tid; (idx / kGroup) < qn; idx += kLdsBlock) {
+        const int u = qr[idx / kGroup];
+        const double du = as_d(dist[u]);
+        const int eb = offs[u], ee = offs[u + 1];
+        if (g == 0) { ++n_exp; n_scan += (unsigned long long)(ee - eb); }
+        for (int e = eb + g; e < ee; e += kGroup) {
+          const int v = A.fw_dst[e];
+          const bool low = relax(du, v, fw[e]);
+          list(v, low && v >= A.ft0);
+        }
+      }
+    } else {
+      for (int slot = tid; slot < 4 * W; slot += kLdsBlock) {
+        const int w = slot >> 2, q = slot & 3;
+        unsigned bits = (unsigned)((cur[w] >> (16 * q)) & 0xFFFFull);
+        while (bits) {
+          const int b = __ffs(bits) - 1;
+          bits &= bits - 1;
+          int u = (w << 6) + 16 * q + b;
+          for (int hops = 0;; ++hops) {
+            const double du = as_d(dist[u]);
+            const int eb = offs[u], ee = offs[u + 1];
+            ++n_exp;
+            n_scan += (unsigned long long)(ee - eb);
+            int next = -1;
+            for (int e = eb; e < ee; ++e) {
+              const int v = A.fw_dst[e];
+              bool cand = relax(du, v, fw[e]) && v >= A.ft0;
+              if (cand && next < 0 && hops < A.chase) { next = v; cand = false; }
+              list(v, cand);
             }
+            if (next < 0) break;
+            u = next;
           }
-          if (next < 0) break;
-          u = next;
         }
       }
     }
     for (int w = tid; w < W; w += kLdsBlock) clr[w] = 0;
-    if (tid == 0) s_more[(r + 1) % 3] = 0;
+    if (tid == 0) s_qn[(r + 1) % 3] = 0;
     __syncthreads();
-    if (!s_more[r % 3]) break;
+    if (!s_qn[r % 3]) break;
     if (r > N + 8) {            // only possible with negative costs
       if (tid == 0 && atomicCAS(&A.err[0], 0, 2) == 0) { A.err[1] = src; A.err[2] = 0; }
       return;
@@ -1227,6 +1274,7 @@ struct tapgpu_ctx {
   std::vector<int> h_tail;            // tail node of each link (0-based)
   bool lds_ok = false;
   size_t lds_bytes = 0;
+  int qcap = 0;                       // LDS frontier queue entries (per buffer)
   // labels
   double* dist = nullptr;
   int *pred = nullptr, *pend = nullptr, *col_src = nullptr, *grp_var = nullptr;
@@ -1490,6 +1538,7 @@ OriginArgs make_origin_args(tapgpu_ctx* c, const unsigned char* var_on) {
   A.od_start = c->od_start; A.od_dest = c->od_dest; A.od_val = c->od_val;
   A.var_on = var_on; A.y = c->y; A.err = c->err_dev; A.counters = c->counters;
   A.chase = c->chase >= 0 ? c->chase : kDefaultChase;
+  A.qcap = c->qcap;
   A.h_dist = c->h_dist; A.h_rnd = c->h_rnd; A.h_pred = c->h_pred; A.h_ptail = c->h_ptail;
   A.delta = c->delta_mult > 0 ? c->delta_mult * c->mean_t0 : 0.0;
   A.z0 = 0;
@@ -1932,10 +1981,16 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
       lds_max = 0;
-    c->lds_bytes = origin_lds_bytes(c->N);
+    c->lds_bytes = origin_lds_bytes(c->N, 0);
     if (hipDeviceGetAttribute(&c->clock_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess)
       c->clock_khz = 0;
     c->lds_ok = c->lds_bytes + 128 <= (size_t)lds_max && E < 65535 && N <= kLdsMaxNodes;
+    if (c->lds_ok) {
+      // the rest of the LDS, up to kLdsQueueMax entries, holds the frontier queues
+      const size_t spare = (size_t)lds_max - 128 - c->lds_bytes;
+      c->qcap = (int)std::min<size_t>(kLdsQueueMax, spare / 4) & ~63;
+      c->lds_bytes = origin_lds_bytes(c->N, c->qcap);
+    }
     c->hbm_lds = 24ull * ((size_t)(N + 63) / 64);
     c->hbm_ok = c->hbm_lds + 256 <= (size_t)lds_max;
     int ncu = 0, per_cu = 0;
