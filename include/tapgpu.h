@@ -217,13 +217,23 @@ int tapgpu_set_profiling(tapgpu_ctx *ctx, int on);
    430k nodes).  tapgpu_get_strategy returns the one in use (1, 2 or 3). */
 int tapgpu_set_strategy(tapgpu_ctx *ctx, int strategy);
 
-/* Worklist schedule of the one-origin-per-workgroup path: how many labels a
-   thread lowers and then expands itself, in a chain, within one round
-   (0: every lowered node waits for the next round).  -1 (the default):
-   tuned on the first loading of the context by timing each candidate
+/* Worklist schedule of the one-origin-per-workgroup path (strategy 2),
+   rounds that scan the frontier bitmap (see tapgpu_set_origin_queue): how
+   many labels a thread lowers and then expands itself, in a chain, within
+   one round (0, the default: every lowered node waits for the next round).
+   -1: tuned on the next loading by timing each candidate
    (0, 2, 4, 8, 16, 32) on the real launch.  Results do not depend on it. */
 int tapgpu_set_origin_chase(tapgpu_ctx *ctx, int chase);
 int tapgpu_get_origin_chase(const tapgpu_ctx *ctx);
+
+/* Frontier queue of the one-origin-per-workgroup path (strategy 2): a round
+   whose frontier (the nodes lowered in the round before) has at most this
+   many nodes expands them from an LDS list, eight lanes per node; a larger
+   frontier is found by scanning its bitmap.  -1 (the default): the most
+   the workgroup's spare LDS holds (up to 4096); 0: bitmap scans only;
+   larger values are clamped to that most.  Results do not depend on it. */
+int tapgpu_set_origin_queue(tapgpu_ctx *ctx, int entries);
+int tapgpu_get_origin_queue(const tapgpu_ctx *ctx);
 
 /* Worklist schedule of the labels-in-HBM path (strategy 3): bucket width of
    its label-ordered rounds, as a multiple of the mean free-flow link time

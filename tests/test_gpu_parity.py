@@ -221,19 +221,29 @@ def _regional_oracle(oracle_lib):
 
 
 def test_schedule_knobs_do_not_change_results(engine_mod):
-    """Chained expansions (LDS kernel) and bucket widths (HBM kernel) are
-    schedules only: volumes bit-identical for every setting; bad values
-    rejected."""
+    """Frontier queue size and chained expansions (LDS kernel) and bucket
+    widths (HBM kernel) are schedules only: volumes bit-identical for every
+    setting; bad values rejected.  Queue 0 runs every round as a bitmap
+    scan; queue 16 mixes queued rounds with bitmap rounds."""
     print("case schedule knobs", flush=True)
     net, dem = network("metro", 0.3)
     out = []
     with engine_mod.TapEngine(net, dem) as eng:
         eng.set_strategy("lds")
-        for ch in (0, 1, 4, 32):
+        qmax = eng.origin_queue
+        assert qmax >= 64
+        for q, ch in ((-1, 0), (0, 0), (0, 1), (0, 4), (0, 32), (16, 0), (16, 4)):
+            eng.set_origin_queue(q)
             eng.set_origin_chase(ch)
             out.append(eng.aon())
+        assert eng.origin_queue == 16
         with pytest.raises(engine_mod.TapGpuError):
             eng.set_origin_chase(-2)
+        with pytest.raises(engine_mod.TapGpuError):
+            eng.set_origin_queue(-2)
+        eng.set_origin_queue(10 ** 6)         # clamped to what the LDS holds
+        assert eng.origin_queue == qmax
+        eng.set_origin_queue(-1)
         eng.set_origin_chase(0)
         for sp in (50, 200):                  # co-run share for the HBM kernel
             eng.set_origin_split(sp)

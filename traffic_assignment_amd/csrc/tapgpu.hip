@@ -431,7 +431,10 @@ constexpr int kLdsPerThread = kLdsMaxNodes / kLdsBlock;
 #endif
 constexpr int kLdsQueueMax = TAPGPU_LDS_QUEUE_MAX;   // frontier queue entries per round
 #ifndef TAPGPU_LDS_GROUP
-#define TAPGPU_LDS_GROUP 8
+#define TAPGPU_LDS_GROUP 6
+#endif
+#ifndef TAPGPU_LDS_PREREAD
+#define TAPGPU_LDS_PREREAD 0
 #endif
 constexpr int kGroup = TAPGPU_LDS_GROUP;             // lanes per expanded node (one link each)
 constexpr unsigned short kNoPred = 0xFFFF;
@@ -529,21 +532,32 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
   //      When more nodes were lowered than the queue holds, the round scans
   //      their bitmap `cur` in 16-bit slots instead, a lane expanding each
   //      set bit's node (and chaining up to A.chase further expansions of
-  //      nodes it lowers itself).  A node lowered in round r is listed once:
-  //      the lane whose atomicOr sets its bit in `nxt` appends it to queue
-  //      r&1, one LDS atomicAdd per wave for all of the wave's appends.
-  //      Three rotating bitmaps: the third one is cleared while the first
-  //      is read.  Labels are the same least fixed point for any such
-  //      order. ----
+  //      nodes it lowers itself).  A node lowered in round r is listed
+  //      once: the lane whose atomicOr sets its bit in `nxt` appends it to
+  //      queue r&1, one LDS atomicAdd per wave for all of the wave's
+  //      appends.  Three rotating bitmaps: the third one is cleared while
+  //      the first is read.  Labels are the same least fixed point for any
+  //      such order. ----
   unsigned long long *cur = bmA, *nxt = bmB, *clr = bmC;
   const int lane = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
   int r = 1;
-  // v was lowered by this lane (cand); list it if this lane set its bit
-  auto list = [&](int v, bool cand) {
+  // relaxes u -> v at cost w and, if this lane lowered v and `want`, lists
+  // it unless listed already; returns whether this lane lowered v
+  auto relax = [&](double du, int v, double w, bool want) {
+    const unsigned long long cb = as_b(du + w);
+#if TAPGPU_LDS_PREREAD
+    const bool low = cb < dist[v] && cb < atomicMin(&dist[v], cb);
+#else
+    const bool low = cb < atomicMin(&dist[v], cb);
+#endif
     bool first = false;
-    if (cand) {
-      const unsigned long long bit = 1ull << (v & 63);
-      first = !(atomicOr(&nxt[v >> 6], bit) & bit);
+    if (low) {
+      ++n_upd;
+      rnd[v] = (unsigned short)r;
+      if (want && v >= A.ft0) {
+        const unsigned long long bit = 1ull << (v & 63);
+        first = !(atomicOr(&nxt[v >> 6], bit) & bit);
+      }
     }
     const unsigned long long bal = __ballot(first);
     if (bal) {
@@ -557,32 +571,21 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
         if (pos < A.qcap) qw_of(r)[pos] = (unsigned short)v;
       }
     }
-  };
-  // relaxes u -> v at cost w; true if this lane lowered v
-  auto relax = [&](double du, int v, double w) {
-    const unsigned long long cb = as_b(du + w);
-    if (cb < dist[v] && cb < atomicMin(&dist[v], cb)) {
-      ++n_upd;
-      rnd[v] = (unsigned short)r;
-      return true;
-    }
-    return false;
+    return low;
   };
   for (;; ++r) {
+    const unsigned short* qr = (r & 1) ? queue0 : queue1;
+    const int j0 = tid / kGroup;
+    const int u0 = j0 < A.qcap ? qr[j0] : 0;      // read beside the count
     const int qn = s_qn[(r - 1) % 3];
     if (qn <= A.qcap) {
-      const unsigned short* qr = (r & 1) ? queue0 : queue1;
-      const int g = tid & (kGroup - 1);
-      for (int idx = tid; (idx / kGroup) < qn; idx += kLdsBlock) {
-        const int u = qr[idx / kGroup];
+      for (int idx = tid; idx / kGroup < qn; idx += kLdsBlock) {
+        const int u = idx == tid ? u0 : qr[idx / kGroup];
+        const int g = idx % kGroup;
         const double du = as_d(dist[u]);
         const int eb = offs[u], ee = offs[u + 1];
         if (g == 0) { ++n_exp; n_scan += (unsigned long long)(ee - eb); }
-        for (int e = eb + g; e < ee; e += kGroup) {
-          const int v = A.fw_dst[e];
-          const bool low = relax(du, v, fw[e]);
-          list(v, low && v >= A.ft0);
-        }
+        for (int e = eb + g; e < ee; e += kGroup) relax(du, A.fw_dst[e], fw[e], true);
       }
     } else {
       for (int slot = tid; slot < 4 * W; slot += kLdsBlock) {
@@ -600,9 +603,9 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
             int next = -1;
             for (int e = eb; e < ee; ++e) {
               const int v = A.fw_dst[e];
-              bool cand = relax(du, v, fw[e]) && v >= A.ft0;
-              if (cand && next < 0 && hops < A.chase) { next = v; cand = false; }
-              list(v, cand);
+              // the first node this lane lowers is chained, not listed
+              const bool chain = next < 0 && hops < A.chase;
+              if (relax(du, v, fw[e], !chain) && chain && v >= A.ft0) next = v;
             }
             if (next < 0) break;
             u = next;
@@ -623,8 +626,14 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
   const int relax_rounds = r;
   const long long clk1 = wall_clock64();
 
-  // ---- predecessor links (label, then round/node key; see above) ----
-  unsigned short my_tail[kLdsPerThread];
+  // ---- predecessor links (label, then round/node key; see above).  The
+  //      in-link offsets are staged in LDS (the out-link offsets are dead);
+  //      each node's tails, costs and link ids are loaded together, so a
+  //      node costs one L2 round trip.  Results stay in registers until
+  //      every thread is done reading offsets and rounds. ----
+  for (int v = tid; v <= N; v += kLdsBlock) offs[v] = (unsigned short)A.rv_off[v];
+  __syncthreads();
+  unsigned short my_tail[kLdsPerThread], my_pred[kLdsPerThread];
   {
     int j = 0;
     for (int v = tid; v < N; v += kLdsBlock, ++j) {
@@ -633,14 +642,13 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
       if (v != src && dv != kMax) {
         unsigned best = 0xFFFFFFFFu;
         unsigned long long best_du = ~0ull;
-        const int kb = A.rv_off[v], ke = A.rv_off[v + 1];
-        // tails and costs of up to 8 in-links are loaded before any is tested
+        const int kb = offs[v], ke = offs[v + 1];
         for (int k0 = kb; k0 < ke; k0 += 8) {
-          int hu[8];
+          int hu[8], he[8];
           double hw[8];
 #pragma unroll
           for (int i = 0; i < 8; ++i)
-            if (k0 + i < ke) { hu[i] = A.rv_src[k0 + i]; hw[i] = rw[k0 + i]; }
+            if (k0 + i < ke) { hu[i] = A.rv_src[k0 + i]; hw[i] = rw[k0 + i]; he[i] = A.rv_eid[k0 + i]; }
 #pragma unroll
           for (int i = 0; i < 8; ++i) {
             if (k0 + i >= ke) break;
@@ -653,26 +661,26 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
             if (du < best_du || (du == best_du && key < best)) {
               best = key;
               best_du = du;
-              p = (unsigned short)A.rv_eid[k0 + i];
+              p = (unsigned short)he[i];
               pt = (unsigned short)u;
             }
           }
         }
       }
       my_tail[j] = pt;
+      my_pred[j] = p;
       if (A.out_dist) {
         const size_t o = (size_t)v * A.ld + blockIdx.x;
         A.out_dist[o] = as_d(dv);
         A.out_pred[o] = (p == kNoPred) ? -1 : (int)p;
       }
-      pred[v] = p;        // offs are no longer needed
     }
   }
-  __syncthreads();        // every rnd[] read is done
+  __syncthreads();        // every offs[] and rnd[] read is done
   const long long clk2 = wall_clock64();
   {
     int j = 0;
-    for (int v = tid; v < N; v += kLdsBlock, ++j) ptail[v] = my_tail[j];
+    for (int v = tid; v < N; v += kLdsBlock, ++j) { pred[v] = my_pred[j]; ptail[v] = my_tail[j]; }
   }
   if (!load) {
     if (A.counters) {
@@ -1252,7 +1260,7 @@ struct tapgpu_ctx {
   double* od_val = nullptr;
   // strategy: 0 auto, 1 node-major sweeps, 2 one origin per workgroup (LDS)
   int strategy = 0;
-  int chase = -1;                     // LDS path: chained expansions (-1: tune)
+  int chase = 0;                      // LDS path: chained expansions (-1: tune)
   int split = 0;                      // LDS path: origins per variant run by the HBM kernel
                                       // alongside it (-1: tune, 0: none)
   hipStream_t stream2 = nullptr;      // second queue for that co-running kernel
@@ -1274,7 +1282,8 @@ struct tapgpu_ctx {
   std::vector<int> h_tail;            // tail node of each link (0-based)
   bool lds_ok = false;
   size_t lds_bytes = 0;
-  int qcap = 0;                       // LDS frontier queue entries (per buffer)
+  int qcap = 0;                       // LDS frontier queue entries in use (per buffer)
+  int qcap_max = 0;                   // entries the LDS allocation holds
   // labels
   double* dist = nullptr;
   int *pred = nullptr, *pend = nullptr, *col_src = nullptr, *grp_var = nullptr;
@@ -1988,8 +1997,9 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
     if (c->lds_ok) {
       // the rest of the LDS, up to kLdsQueueMax entries, holds the frontier queues
       const size_t spare = (size_t)lds_max - 128 - c->lds_bytes;
-      c->qcap = (int)std::min<size_t>(kLdsQueueMax, spare / 4) & ~63;
-      c->lds_bytes = origin_lds_bytes(c->N, c->qcap);
+      c->qcap_max = (int)std::min<size_t>(kLdsQueueMax, spare / 4) & ~63;
+      c->qcap = c->qcap_max;
+      c->lds_bytes = origin_lds_bytes(c->N, c->qcap_max);
     }
     c->hbm_lds = 24ull * ((size_t)(N + 63) / 64);
     c->hbm_ok = c->hbm_lds + 256 <= (size_t)lds_max;
@@ -2549,6 +2559,17 @@ int tapgpu_get_origin_split(const tapgpu_ctx* c) { return c ? c->split : TAPGPU_
 int tapgpu_get_origin_chase(const tapgpu_ctx* c) {
   if (!c) return TAPGPU_E_ARG;
   return c->chase;
+}
+
+int tapgpu_set_origin_queue(tapgpu_ctx* c, int entries) {
+  if (!c || entries < -1) return TAPGPU_E_ARG;
+  c->qcap = entries < 0 ? c->qcap_max : std::min(entries, c->qcap_max);
+  return TAPGPU_OK;
+}
+
+int tapgpu_get_origin_queue(const tapgpu_ctx* c) {
+  if (!c) return TAPGPU_E_ARG;
+  return c->qcap;
 }
 
 int tapgpu_get_strategy(const tapgpu_ctx* c) {

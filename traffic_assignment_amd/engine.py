@@ -117,6 +117,8 @@ def lib():
         L.tapgpu_get_strategy.argtypes = [vp]
         L.tapgpu_set_origin_chase.argtypes = [vp, C.c_int]
         L.tapgpu_get_origin_chase.argtypes = [vp]
+        L.tapgpu_set_origin_queue.argtypes = [vp, C.c_int]
+        L.tapgpu_get_origin_queue.argtypes = [vp]
         L.tapgpu_set_origin_delta.argtypes = [vp, C.c_double]
         L.tapgpu_set_origin_split.argtypes = [vp, C.c_int]
         L.tapgpu_greedy.argtypes = [vp, C.c_int64, C.c_int, vp, vp, vp, vp, C.c_int,
@@ -295,12 +297,22 @@ class TapEngine:
 
     def set_origin_chase(self, chase):
         """Chained expansions per thread and round on the one-origin-per-
-        workgroup path (-1: tune on the next loading)."""
+        workgroup path, in rounds that scan the frontier bitmap (-1: tune on
+        the next loading)."""
         self._check(lib().tapgpu_set_origin_chase(self._h, int(chase)))
 
     @property
     def origin_chase(self):
         return lib().tapgpu_get_origin_chase(self._h)
+
+    def set_origin_queue(self, entries):
+        """Largest frontier the one-origin-per-workgroup path expands from
+        its LDS queue (-1: as many as its LDS holds, 0: bitmap scans only)."""
+        self._check(lib().tapgpu_set_origin_queue(self._h, int(entries)))
+
+    @property
+    def origin_queue(self):
+        return lib().tapgpu_get_origin_queue(self._h)
 
     def set_origin_split(self, origins):
         """Origins per variant co-run by the labels-in-HBM kernel beside the
