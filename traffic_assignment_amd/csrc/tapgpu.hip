@@ -450,6 +450,7 @@ struct OriginArgs {
   const unsigned char* var_on;       // [V] or null
   int chase;                         // expansions a thread chains per round
   int qcap;                          // LDS frontier queue entries per round (0: bitmap scan only)
+  int hqcap;                         // labels-in-HBM kernel: queue entries per round (0: bitmap scan only)
   double delta;                      // labels-in-HBM kernel: bucket width (<= 0: none)
   int z0, zn;                        // labels-in-HBM kernel: origins [z0, z0 + zn) of each variant
   double* y;                         // [V * E], zeroed by the caller
@@ -777,7 +778,27 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
 // the lowered node is queued for the next round, so the fixed point is the
 // same as with strict rounds.
 
-constexpr int kHbmBlock = 256;                 // 4 workgroups per CU: more origins in flight
+#ifndef TAPGPU_HBM_BLOCK
+#define TAPGPU_HBM_BLOCK 512
+#endif
+constexpr int kHbmBlock = TAPGPU_HBM_BLOCK;    // 512: 2 workgroups per CU, each with room for queues
+#ifndef TAPGPU_HBM_PER_CU
+#define TAPGPU_HBM_PER_CU (1024 / TAPGPU_HBM_BLOCK)
+#endif
+constexpr int kHbmPerCu = TAPGPU_HBM_PER_CU;    // workgroups sharing a CU's LDS (registers: <= 128 VGPRs)
+constexpr int kHbmQueueMax = 4096;              // frontier queue entries per round
+#ifndef TAPGPU_HBM_FRESH
+#define TAPGPU_HBM_FRESH 1
+#endif
+#ifndef TAPGPU_HBM_PREREAD
+#define TAPGPU_HBM_PREREAD 1
+#endif
+
+// LDS of the labels-in-HBM kernel: three bitmaps 3*8W | two queues of
+// (label 8 B, node 4 B) entries 2*12Q.
+size_t origin_hbm_lds_bytes(int N, int hqcap) {
+  return 24ull * (size_t)((N + 63) / 64) + 24ull * (size_t)hqcap;
+}
 
 // workgroup-scope loads of a workgroup's private scratch row
 __device__ __forceinline__ unsigned long long ldg_u64(const unsigned long long* p) {
@@ -797,7 +818,12 @@ __global__ __launch_bounds__(kHbmBlock) void k_origin_hbm(OriginArgs A, int load
   unsigned long long* bmA = sm;
   unsigned long long* bmB = bmA + W;
   unsigned long long* bmC = bmB + W;
-  __shared__ int s_more[3], s_exp[3];
+  unsigned long long* qlab0 = bmC + W;                  // 2 x [hqcap] labels when listed
+  unsigned long long* qlab1 = qlab0 + A.hqcap;
+  unsigned* qnode0 = (unsigned*)(qlab1 + A.hqcap);      // 2 x [hqcap] nodes
+  unsigned* qnode1 = qnode0 + A.hqcap;
+  const int lane = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+  __shared__ int s_qn[3], s_exp[3];
   __shared__ unsigned long long s_minq[3];
   __shared__ unsigned long long s_cnt[8];
   __shared__ int s_maxlen;
@@ -826,84 +852,135 @@ __global__ __launch_bounds__(kHbmBlock) void k_origin_hbm(OriginArgs A, int load
       __hip_atomic_store(&rnd[v], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     for (int w = tid; w < 3 * W; w += kHbmBlock) bmA[w] = 0;
-    if (tid < 3) { s_more[tid] = 0; s_exp[tid] = 0; s_minq[tid] = ~0ull; }
+    if (tid < 3) { s_qn[tid] = tid == 0 ? 1 : 0; s_exp[tid] = 0; s_minq[tid] = ~0ull; }
     __syncthreads();
     if (tid == 0) {
       __hip_atomic_store(&dist[src], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       bmA[src >> 6] = 1ull << (src & 63);
+      if (A.hqcap > 0) { qnode0[0] = (unsigned)src; qlab0[0] = 0ull; }
     }
     __syncthreads();
 
     // ---- label-correcting rounds, bucketed: a round expands the queued
     //      nodes whose label is below T and keeps the others queued; a
     //      round that expands nothing moves T to the smallest queued label
-    //      plus delta (delta <= 0: no buckets, every queued node expands) ----
+    //      plus delta (delta <= 0: no buckets, every queued node expands).
+    //      The nodes queued for round r (lowered or kept in round r-1) are
+    //      listed in an LDS queue with their label when listed (count
+    //      s_qn[(r-1)%3], buffer (r-1)&1), one node per thread, so keeping a
+    //      node needs no HBM read; the label only decides the bucket (the
+    //      current label, never higher, is read to expand).  When more were
+    //      queued than the queue holds, the round scans their bitmap `cur`
+    //      instead.  A node is listed once per round: the lane whose
+    //      atomicOr sets its bit in `nxt` appends it, one LDS atomicAdd per
+    //      wave for all of the wave's appends. ----
     unsigned long long *cur = bmA, *nxt = bmB, *clr = bmC;
     const bool buckets = A.delta > 0.0;
     double T = buckets ? A.delta : INFINITY;
     int r = 1;
+    // sets v's bit in nxt; the lane that set it appends (v, lab)
+    auto list = [&](bool want, int v, unsigned long long lab) {
+      bool first = false;
+      if (want) {
+        const unsigned long long bit = 1ull << (v & 63);
+        first = !(atomicOr(&nxt[v >> 6], bit) & bit);
+      }
+      const unsigned long long bal = __ballot(first);
+      if (bal) {
+        const int leader = __ffsll((long long)bal) - 1;
+        int b0 = 0;
+        if (lane == leader) b0 = atomicAdd(&s_qn[r % 3], __popcll(bal));
+        b0 = __builtin_amdgcn_readlane(b0, leader);
+        if (first) {
+          const int pos = b0 + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+          if (pos < A.hqcap) {
+            ((r & 1) ? qlab1 : qlab0)[pos] = lab;
+            ((r & 1) ? qnode1 : qnode0)[pos] = (unsigned)v;
+          }
+        }
+      }
+    };
+    // u (listed with label lab) either stays queued (lab >= T) or expands
+    auto visit = [&](int u, unsigned long long lab) {
+      const int sl = r % 3;
+#if TAPGPU_HBM_FRESH
+      if (!(as_d(lab) < T)) lab = ldg_u64(&dist[u]);
+#endif
+      if (!(as_d(lab) < T)) {                 // later bucket: stays queued
+        atomicMin(&s_minq[sl], lab);
+        ++n_def;
+        list(true, u, lab);
+        return;
+      }
+      s_exp[sl] = 1;
+      const double du = as_d(ldg_u64(&dist[u]));
+      const int eb = A.fw_off[u], ee = A.fw_off[u + 1];
+      ++n_exp;
+      n_scan += (unsigned long long)(ee - eb);
+      for (int e0 = eb; e0 < ee; e0 += 8) {
+        int hv[8];
+        double hw[8];
+        unsigned long long hd[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (e0 + k < ee) { hv[k] = A.fw_dst[e0 + k]; hw[k] = fw[e0 + k]; }
+#if TAPGPU_HBM_PREREAD
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (e0 + k < ee) hd[k] = ldg_u64(&dist[hv[k]]);
+#endif
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          if (e0 + k >= ee) break;
+          const int v = hv[k];
+          const unsigned long long cb = as_b(du + hw[k]);
+          bool low = false;
+#if TAPGPU_HBM_PREREAD
+          if (cb < hd[k]) {
+#else
+          {
+#endif
+            const unsigned long long old =
+                __hip_atomic_fetch_min(&dist[v], cb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (cb < old) {
+              ++n_upd;
+              __hip_atomic_store(&rnd[v], (unsigned)r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              low = v >= A.ft0;
+            }
+          }
+          list(low, v, cb);
+        }
+      }
+    };
     for (;; ++r) {
       const int sl = r % 3;
-      for (int slot = tid; slot < 4 * W; slot += kHbmBlock) {
-        const int w = slot >> 2, q = slot & 3;
-        unsigned bits = (unsigned)((cur[w] >> (16 * q)) & 0xFFFFull);
-        while (bits) {
-          const int b = __ffs(bits) - 1;
-          bits &= bits - 1;
-          const int u = (w << 6) + 16 * q + b;
-          const unsigned long long dub = ldg_u64(&dist[u]);
-          const double du = as_d(dub);
-          if (!(du < T)) {                      // later bucket: stays queued
-            atomicOr(&nxt[u >> 6], 1ull << (u & 63));
-            atomicMin(&s_minq[sl], dub);
-            s_more[sl] = 1;
-            ++n_def;
-            continue;
-          }
-          s_exp[sl] = 1;
-          const int eb = A.fw_off[u], ee = A.fw_off[u + 1];
-          ++n_exp;
-          n_scan += (unsigned long long)(ee - eb);
-          for (int e0 = eb; e0 < ee; e0 += 8) {
-            int hv[8];
-            double hw[8];
-            unsigned long long hd[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-              if (e0 + k < ee) { hv[k] = A.fw_dst[e0 + k]; hw[k] = fw[e0 + k]; }
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-              if (e0 + k < ee) hd[k] = ldg_u64(&dist[hv[k]]);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-              if (e0 + k >= ee) break;
-              const int v = hv[k];
-              const unsigned long long cb = as_b(du + hw[k]);
-              if (cb < hd[k]) {
-                const unsigned long long old =
-                    __hip_atomic_fetch_min(&dist[v], cb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                if (cb < old) {
-                  ++n_upd;
-                  __hip_atomic_store(&rnd[v], (unsigned)r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                  if (v >= A.ft0) {
-                    atomicOr(&nxt[v >> 6], 1ull << (v & 63));
-                    s_more[sl] = 1;
-                  }
-                }
-              }
-            }
+      const int qn = s_qn[(r - 1) % 3];
+      if (qn <= A.hqcap) {
+        const unsigned* qn_r = (r & 1) ? qnode0 : qnode1;
+        const unsigned long long* ql_r = (r & 1) ? qlab0 : qlab1;
+        for (int i = tid; i < qn; i += kHbmBlock) visit((int)qn_r[i], ql_r[i]);
+      } else {
+        for (int slot = tid; slot < 4 * W; slot += kHbmBlock) {
+          const int w = slot >> 2, q = slot & 3;
+          unsigned bits = (unsigned)((cur[w] >> (16 * q)) & 0xFFFFull);
+          while (bits) {
+            const int b = __ffs(bits) - 1;
+            bits &= bits - 1;
+            const int u = (w << 6) + 16 * q + b;
+            visit(u, ldg_u64(&dist[u]));
           }
         }
       }
       for (int w = tid; w < W; w += kHbmBlock) clr[w] = 0;
       if (tid == 0) {
         const int sn = (r + 1) % 3;
-        s_more[sn] = 0;
+        s_qn[sn] = 0;
         s_exp[sn] = 0;
         s_minq[sn] = ~0ull;
       }
-        __syncthreads();
-      if (!s_more[sl]) break;
+      __syncthreads();
+      if (!s_qn[sl]) break;
       if (buckets && !s_exp[sl]) T = as_d(s_minq[sl]) + A.delta;
       if (r > 64 * N + 64) {
         if (tid == 0 && atomicCAS(&A.err[0], 0, 2) == 0) { A.err[1] = src; A.err[2] = 0; }
@@ -914,23 +991,23 @@ __global__ __launch_bounds__(kHbmBlock) void k_origin_hbm(OriginArgs A, int load
     n_rounds += (unsigned long long)r;
     const long long clk1 = wall_clock64();
 
-    // ---- predecessor links: smallest tail label, then (round, node) ----
+    // ---- predecessor links: smallest tail label, then (round, node).
+    //      Tails, costs and link ids of up to 4 in-links are loaded
+    //      together, then the tails' labels; rounds are read only to break
+    //      an exact tie of labels. ----
     for (int v = tid; v < N; v += kHbmBlock) {
       int p = -1, pt = -1;
       const unsigned long long dv = ldg_u64(&dist[v]);
       if (v != src && dv != kMax) {
-        unsigned long long best = ~0ull, best_du = ~0ull;
+        unsigned long long best_du = ~0ull;
         const int kb = A.rv_off[v], ke = A.rv_off[v + 1];
-        // tails, costs and tail labels of up to 4 in-links are loaded
-        // before any is tested (4, not 8: registers decide how many
-        // workgroups a CU holds)
         for (int k0 = kb; k0 < ke; k0 += 4) {
-          int hu[4];
+          int hu[4], he[4];
           double hw[4];
           unsigned long long hd[4];
 #pragma unroll
           for (int i = 0; i < 4; ++i)
-            if (k0 + i < ke) { hu[i] = A.rv_src[k0 + i]; hw[i] = rw[k0 + i]; }
+            if (k0 + i < ke) { hu[i] = A.rv_src[k0 + i]; hw[i] = rw[k0 + i]; he[i] = A.rv_eid[k0 + i]; }
 #pragma unroll
           for (int i = 0; i < 4; ++i)
             if (k0 + i < ke) hd[i] = ldg_u64(&dist[hu[i]]);
@@ -942,11 +1019,15 @@ __global__ __launch_bounds__(kHbmBlock) void k_origin_hbm(OriginArgs A, int load
             const unsigned long long du = hd[i];
             if (du == kMax) continue;
             if (as_b(as_d(du) + hw[i]) != dv) continue;
-            const unsigned long long key = ((unsigned long long)ldg_u32(&rnd[u]) << 32) | (unsigned)u;
-            if (du < best_du || (du == best_du && key < best)) {
-              best = key;
+            bool take = du < best_du;
+            if (!take && du == best_du) {
+              const unsigned long long key = ((unsigned long long)ldg_u32(&rnd[u]) << 32) | (unsigned)u;
+              const unsigned long long bkey = ((unsigned long long)ldg_u32(&rnd[pt]) << 32) | (unsigned)pt;
+              take = key < bkey;
+            }
+            if (take) {
               best_du = du;
-              p = A.rv_eid[k0 + i];
+              p = he[i];
               pt = u;
             }
           }
@@ -1284,6 +1365,7 @@ struct tapgpu_ctx {
   size_t lds_bytes = 0;
   int qcap = 0;                       // LDS frontier queue entries in use (per buffer)
   int qcap_max = 0;                   // entries the LDS allocation holds
+  int hqcap = 0;                      // labels-in-HBM kernel: queue entries (per buffer)
   // labels
   double* dist = nullptr;
   int *pred = nullptr, *pend = nullptr, *col_src = nullptr, *grp_var = nullptr;
@@ -1548,6 +1630,7 @@ OriginArgs make_origin_args(tapgpu_ctx* c, const unsigned char* var_on) {
   A.var_on = var_on; A.y = c->y; A.err = c->err_dev; A.counters = c->counters;
   A.chase = c->chase >= 0 ? c->chase : kDefaultChase;
   A.qcap = c->qcap;
+  A.hqcap = c->hqcap;
   A.h_dist = c->h_dist; A.h_rnd = c->h_rnd; A.h_pred = c->h_pred; A.h_ptail = c->h_ptail;
   A.delta = c->delta_mult > 0 ? c->delta_mult * c->mean_t0 : 0.0;
   A.z0 = 0;
@@ -2001,8 +2084,19 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
       c->qcap = c->qcap_max;
       c->lds_bytes = origin_lds_bytes(c->N, c->qcap_max);
     }
-    c->hbm_lds = 24ull * ((size_t)(N + 63) / 64);
+    c->hbm_lds = origin_hbm_lds_bytes(N, 0);
     c->hbm_ok = c->hbm_lds + 256 <= (size_t)lds_max;
+    if (c->hbm_ok) {
+      // queues from the LDS share of one of the kHbmPerCu workgroups on a CU
+      int lds_cu = 0;
+      if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
+              hipSuccess || lds_cu <= 0)
+        lds_cu = lds_max;
+      const size_t share = std::min<size_t>((size_t)lds_cu / kHbmPerCu, (size_t)lds_max) - 256;
+      const size_t spare = share > c->hbm_lds ? share - c->hbm_lds : 0;
+      c->hqcap = (int)std::min<size_t>(kHbmQueueMax, spare / 24) & ~63;
+      c->hbm_lds = origin_hbm_lds_bytes(N, c->hqcap);
+    }
     int ncu = 0, per_cu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) ncu = 1;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_origin_hbm, kHbmBlock, c->hbm_lds) !=
