@@ -168,6 +168,55 @@ def test_fw_against_reference_golden(engine_mod, oracle_lib, name, scale, fix, s
     assert np.array_equal(x, o["x"]) and np.array_equal(c, o["c"])
 
 
+def test_fw_line_search_beyond_first_window(engine_mod, oracle_lib):
+    """Past iteration ~7,600 on grid5 the line search needs steps below
+    2^-15, beyond the first window of 16 halvings: the second window is
+    queued with every iteration, later ones are run by the host.  8,450
+    iterations (ten such steps) bit-identical to the oracle; in a batched
+    engine whose second variant has other capacities, the variants need the
+    extra windows at different iterations and each still equals its plain
+    solve."""
+    import dataclasses
+    from traffic_assignment_amd import synth
+    print("case fw deep line search grid5", flush=True)
+    it = 8450
+    net, dem = network("grid5")
+    with engine_mod.TapEngine(net, dem) as eng:
+        eng.fw_init()
+        r = eng.fw_run(max_iters=it)[0]
+        x, c = eng.flows()
+        tr = eng.trace()
+    assert r["iterations"] == it
+    assert np.count_nonzero(tr["lambda_"] < 2.0 ** -15) >= 5
+    oracle_lib.lib().ora_set_pow_mode(1)
+    try:
+        o = oracle_lib.fw_solve(net, dem, max_iters=it, nthreads=8)
+    finally:
+        oracle_lib.lib().ora_set_pow_mode(0)
+    assert np.array_equal(tr["lambda_"], o["trace"]["lambda_"])
+    assert np.array_equal(x, o["x"]) and np.array_equal(c, o["c"])
+    # two variants: the base capacities and 0.8x on every link
+    E = net.num_links
+    cap2 = np.stack([net.capacity, 0.8 * net.capacity])
+    rec = {k: np.stack([getattr(net, k)] * 2) for k in ("fftime", "B", "power")}
+    with engine_mod.TapEngine(net, dem, num_variants=2, present=np.ones((2, E), np.uint8),
+                              capacity=cap2, **rec) as be:
+        be.fw_init()
+        res = be.fw_run(max_iters=it)
+        bx = [be.flows(v)[0] for v in range(2)]
+        blam = [be.trace(v)["lambda_"] for v in range(2)]
+    assert [q["iterations"] for q in res] == [it, it]
+    assert np.array_equal(blam[0], tr["lambda_"]) and np.array_equal(bx[0], x)
+    net2 = dataclasses.replace(net, capacity=0.8 * net.capacity)
+    with engine_mod.TapEngine(net2, dem) as pe:
+        pe.fw_init()
+        pe.fw_run(max_iters=it)
+        assert np.array_equal(blam[1], pe.trace()["lambda_"])
+        assert np.array_equal(bx[1], pe.flows()[0])
+    deep = [set(np.nonzero(l < 2.0 ** -15)[0]) for l in blam]
+    assert deep[1] and deep[0] != deep[1]
+
+
 @pytest.mark.parametrize("strategy", STRATS)
 def test_fw_relgap_target_and_resume(engine_mod, oracle_lib, strategy):
     """Stop rule (relgap > target is the loop test) and running in two

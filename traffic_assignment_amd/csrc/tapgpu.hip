@@ -1278,7 +1278,7 @@ __global__ __launch_bounds__(kBlock) void k_link_decide(VarState* st, const tap_
 // x += lambda * (y - x); c = BPR(x); reverse-order costs for the next sweep.
 __global__ __launch_bounds__(kBlock) void k_link_step(Links K, VarState* st) {
   const int var = blockIdx.y;
-  if (!st[var].stepping) return;
+  if (!st[var].stepping || st[var].need_more) return;     // no step until lambda is known
   const double lambda = st[var].lambda;
   const size_t off = (size_t)var * K.E;
   for (int k = blockIdx.x * kBlock + threadIdx.x; k < K.E; k += gridDim.x * kBlock) {
@@ -1297,6 +1297,7 @@ __global__ __launch_bounds__(kBlock) void k_link_step(Links K, VarState* st) {
 __global__ void k_finish_step(VarState* st, int V, unsigned char* var_on) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= V) return;
+  if (st[v].stepping && st[v].need_more) return;          // still searching lambda
   st[v].stepping = 0;
   var_on[v] = (unsigned char)st[v].active;
 }
@@ -1386,6 +1387,18 @@ struct tapgpu_ctx {
   bool initialised = false;
   // pinned host scratch
   int* h_int = nullptr;
+  // pinned readback of one FW iteration: error flag, origin-kernel
+  // counters, variant states and flags (one synchronisation per iteration)
+  int* pin_err = nullptr;                   // [4]
+  unsigned long long* pin_cnt = nullptr;    // [16]
+  VarState* pin_st = nullptr;               // [V]
+  unsigned char* pin_on = nullptr;          // [V]
+  void* pin_block = nullptr;
+  // loading whose counters and error flag are read at the next
+  // synchronisation instead of right after the launch
+  bool defer_origin = false;
+  bool origin_pending = false;
+  int pending_Vn = 0;
   // stats
   tapgpu_stats stats{};
   bool profiling = false;
@@ -1743,13 +1756,38 @@ int tune_chase(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
   return TAPGPU_OK;
 }
 
-// Reads the error flag and counters of an origin-kernel launch into stats.
+int absorb_origin(tapgpu_ctx* c, const int* herr, const unsigned long long* hc, int Vn,
+                  std::chrono::steady_clock::time_point t0);
+
+// Reads the error flag and counters of an origin-kernel launch into stats:
+// now, or (defer_origin) into pinned memory, absorbed by absorb_pending
+// after the caller's next synchronisation.
 int finish_origin(tapgpu_ctx* c, int Vn, std::chrono::steady_clock::time_point t0) {
+  if (c->defer_origin) {
+    HIPCHK(c, hipMemcpyAsync(c->pin_err, c->err_dev, sizeof(int) * 3, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->pin_cnt, c->counters, sizeof(unsigned long long) * 16,
+                             hipMemcpyDeviceToHost, c->stream));
+    c->origin_pending = true;
+    c->pending_Vn = Vn;
+    return TAPGPU_OK;
+  }
   int herr[3];
   unsigned long long hc[16];
   HIPCHK(c, hipMemcpyAsync(herr, c->err_dev, sizeof herr, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(hc, c->counters, sizeof hc, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return absorb_origin(c, herr, hc, Vn, t0);
+}
+
+// The deferred readback of finish_origin, after the stream has synchronised.
+int absorb_pending(tapgpu_ctx* c) {
+  if (!c->origin_pending) return TAPGPU_OK;
+  c->origin_pending = false;
+  return absorb_origin(c, c->pin_err, c->pin_cnt, c->pending_Vn, std::chrono::steady_clock::now());
+}
+
+int absorb_origin(tapgpu_ctx* c, const int* herr, const unsigned long long* hc, int Vn,
+                  std::chrono::steady_clock::time_point t0) {
   c->stats.origin_expansions += (int64_t)hc[0];
   c->stats.origin_scanned += (int64_t)hc[1];
   c->stats.origin_updates += (int64_t)hc[2];
@@ -2146,6 +2184,15 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
   HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
   HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
   HIPCHK(c, hipHostMalloc((void**)&c->h_int, 64 * sizeof(int), hipHostMallocDefault));
+  {
+    const size_t bytes = 16 + 128 + sizeof(VarState) * (size_t)V + (size_t)V;
+    HIPCHK(c, hipHostMalloc(&c->pin_block, bytes, hipHostMallocDefault));
+    unsigned char* b = (unsigned char*)c->pin_block;
+    c->pin_err = (int*)b;
+    c->pin_cnt = (unsigned long long*)(b + 16);
+    c->pin_st = (VarState*)(b + 144);
+    c->pin_on = b + 144 + sizeof(VarState) * (size_t)V;
+  }
   int rc;
 #define ALLOC(ptr, n) if ((rc = dalloc(c, &c->ptr, (n)))) return rc
   ALLOC(fw_off, N + 1); ALLOC(fw_dst, E); ALLOC(rv_off, N + 1); ALLOC(rv_src, E);
@@ -2242,6 +2289,7 @@ void tapgpu_destroy(tapgpu_ctx* c) {
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->trace) (void)hipFree(c->trace);
   if (c->h_int) (void)hipHostFree(c->h_int);
+  if (c->pin_block) (void)hipHostFree(c->pin_block);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -2472,6 +2520,14 @@ int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_res
   }
   if ((rc = upload(c, c->st, hs.data(), c->V))) return rc;
   if ((rc = upload(c, c->var_on, hon.data(), c->V))) return rc;
+  // One synchronisation per iteration: the loading's error flag and
+  // counters, the variants' states and flags are read back together at the
+  // end.  A second line-search window pass is always queued and does
+  // nothing for variants whose lambda the first window decided; a variant
+  // that needs more windows skips the step and is finished by the host loop
+  // below (rare: lambda below 2^-15).
+  if (!c->ev[2])
+    for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
   for (;;) {
     bool any = false;
     for (int v = 0; v < c->V; ++v) any = any || hon[v];
@@ -2481,33 +2537,61 @@ int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_res
       for (int v = 0; v < c->V; ++v) maxit = std::max(maxit, hs[v].iter);
       if (maxit + 1 >= c->trace_cap && (rc = ensure_trace(c, 2 * c->trace_cap))) return rc;
     }
+    auto t_it = std::chrono::steady_clock::now();
     hipLaunchKernelGGL(k_begin_step, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->st, c->V,
                        c->var_on);
-    if ((rc = load_variants(c, 0, c->V, c->var_on))) return rc;
-    auto tl = std::chrono::steady_clock::now();
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    c->defer_origin = true;
+    rc = load_variants(c, 0, c->V, c->var_on);
+    c->defer_origin = false;
+    if (rc) return rc;
+    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
     hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kBlock), 0, c->stream, K, c->st, c->part, 0);
     hipLaunchKernelGGL(k_link_decide, dim3(c->V), dim3(kBlock), 0, c->stream, c->st, c->part,
                        kLinkBlocks, 0, c->trace, c->trace_cap, max_iters, target_relgap);
-    c->stats.link_passes++;
-    for (int extra = 0;; ++extra) {
-      HIPCHK(c, hipMemcpyAsync(hs.data(), c->st, sizeof(VarState) * c->V, hipMemcpyDeviceToHost,
-                               c->stream));
-      HIPCHK(c, hipStreamSynchronize(c->stream));
-      bool more = false;
-      for (int v = 0; v < c->V; ++v) more = more || (hs[v].stepping && hs[v].need_more);
-      if (!more) break;
-      hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kBlock), 0, c->stream, K, c->st, c->part, 1);
-      hipLaunchKernelGGL(k_link_decide, dim3(c->V), dim3(kBlock), 0, c->stream, c->st, c->part,
-                         kLinkBlocks, 1, c->trace, c->trace_cap, max_iters, target_relgap);
-      c->stats.link_passes++;
-    }
+    hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kBlock), 0, c->stream, K, c->st, c->part, 1);
+    hipLaunchKernelGGL(k_link_decide, dim3(c->V), dim3(kBlock), 0, c->stream, c->st, c->part,
+                       kLinkBlocks, 1, c->trace, c->trace_cap, max_iters, target_relgap);
     hipLaunchKernelGGL(k_link_step, lgrid, dim3(kBlock), 0, c->stream, K, c->st);
     hipLaunchKernelGGL(k_finish_step, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->st,
                        c->V, c->var_on);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(hon.data(), c->var_on, c->V, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->pin_st, c->st, sizeof(VarState) * c->V, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->pin_on, c->var_on, c->V, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->stats.link_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tl).count();
+    if ((rc = absorb_pending(c))) return rc;
+    c->stats.link_passes += 1;
+    auto need_more = [&]() {
+      bool more = false;
+      for (int v = 0; v < c->V; ++v) more = more || (c->pin_st[v].stepping && c->pin_st[v].need_more);
+      return more;
+    };
+    bool more = need_more();
+    if (more) c->stats.link_passes += 1;              // the queued second window was used
+    while (more) {
+      hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kBlock), 0, c->stream, K, c->st, c->part, 1);
+      hipLaunchKernelGGL(k_link_decide, dim3(c->V), dim3(kBlock), 0, c->stream, c->st, c->part,
+                         kLinkBlocks, 1, c->trace, c->trace_cap, max_iters, target_relgap);
+      hipLaunchKernelGGL(k_link_step, lgrid, dim3(kBlock), 0, c->stream, K, c->st);
+      hipLaunchKernelGGL(k_finish_step, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->st,
+                         c->V, c->var_on);
+      HIPCHK(c, hipGetLastError());
+      HIPCHK(c, hipMemcpyAsync(c->pin_st, c->st, sizeof(VarState) * c->V, hipMemcpyDeviceToHost,
+                               c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->pin_on, c->var_on, c->V, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      c->stats.link_passes += 1;
+      more = need_more();
+    }
+    std::copy(c->pin_st, c->pin_st + c->V, hs.begin());
+    std::copy(c->pin_on, c->pin_on + c->V, hon.begin());
+    float aon = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&aon, c->ev[2], c->ev[3]));
+    const double it_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_it).count();
+    c->stats.aon_ms += aon;
+    c->stats.link_ms += std::max(0.0, it_ms - aon);
   }
   c->stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   if (results) {
