@@ -291,8 +291,8 @@ def main():
                     help="batched: variants solved one by one for the plain comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--strategy", default="auto", choices=["auto", "sweeps", "lds", "hbm"])
-    ap.add_argument("--chase", type=int, default=-1,
-                    help="LDS path: chained expansions per round (-1: tuned)")
+    ap.add_argument("--chase", type=int, default=0,
+                    help="LDS path: chained expansions per bitmap-scan round (-1: tuned)")
     ap.add_argument("--delta", type=float, default=-1.0,
                     help="HBM path: bucket width in mean free-flow times (-1: tuned)")
     ap.add_argument("--max-iters", type=int, default=0,
@@ -370,7 +370,8 @@ def main():
                    "zones": net.num_zones, "parallelism": "1 GPU"},
         "sssp_gbps": round(achieved, 2),
         "schedule": {"origin_chase": acc.get("origin_chase", 0) // max(args.steps, 1),
-                     "origin_delta": acc.get("origin_delta", 0.0) / max(args.steps, 1)},
+                     "origin_delta": acc.get("origin_delta", 0.0) / max(args.steps, 1),
+                     "origin_queue": eng.origin_queue},
         "origin_phase_us_per_workgroup": {
             k: round(1e3 * acc.get("origin_" + k + "_ms", 0.0) / max(acc.get("origin_blocks", 0), 1), 2)
             for k in ("relax", "pred", "load")},
