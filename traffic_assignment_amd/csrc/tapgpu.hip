@@ -1146,22 +1146,29 @@ __global__ __launch_bounds__(kBlock) void k_costs(Links K, int mode, const unsig
   }
 }
 
-__device__ void block_reduce_acc(tap_acc* v, int nq, tap_acc* sh /* [kBlock] */) {
-  // fixed-order tree over the block, one quantity at a time
-  for (int q = 0; q < nq; ++q) {
-    sh[threadIdx.x] = v[q];
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-      if ((int)threadIdx.x < s) {
-        tap_acc a = sh[threadIdx.x];
-        tap_acc_merge(&a, sh[threadIdx.x + s]);
-        sh[threadIdx.x] = a;
-      }
-      __syncthreads();
+// All kQ compensated sums of a block at once, in a fixed order: a butterfly
+// inside each wavefront (lane 0's result is kept), then thread 0 merges the
+// wavefronts in order.  One barrier; the result is in thread 0's v[].
+__device__ void block_reduce_acc(tap_acc* v, tap_acc* sh /* [kBlock / kWave][kQ] */) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
+  for (int q = 0; q < kQ; ++q) {
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      tap_acc o;
+      o.s = __shfl_xor(v[q].s, off);
+      o.c = __shfl_xor(v[q].c, off);
+      tap_acc_merge(&v[q], o);
     }
-    if (threadIdx.x == 0) v[q] = sh[0];
-    __syncthreads();
   }
+  if (lane == 0)
+    for (int q = 0; q < kQ; ++q) sh[w * kQ + q] = v[q];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int q = 0; q < kQ; ++q) {
+      tap_acc a = sh[q];
+      for (int j = 1; j < kBlock / kWave; ++j) tap_acc_merge(&a, sh[j * kQ + q]);
+      v[q] = a;
+    }
 }
 
 // Partial sums per (variant, block): g0 = sum c*d, objective, vht and
@@ -1194,8 +1201,8 @@ __global__ __launch_bounds__(kBlock) void k_link_partials(Links K, const VarStat
       tap_acc_add(&acc[3 + j], tap_bpr(t0, cap, B, pw, q) * d);
     }
   }
-  __shared__ tap_acc sh[kBlock];
-  block_reduce_acc(acc, kQ, sh);
+  __shared__ tap_acc sh[kBlock / kWave * kQ];
+  block_reduce_acc(acc, sh);
   if (threadIdx.x == 0) {
     tap_acc* out = part + ((size_t)var * gridDim.x + blockIdx.x) * kQ;
     for (int q = 0; q < kQ; ++q) out[q] = acc[q];
@@ -1218,8 +1225,8 @@ __global__ __launch_bounds__(kBlock) void k_link_decide(VarState* st, const tap_
     const tap_acc* in = part + ((size_t)var * nparts + b) * kQ;
     for (int q = 0; q < kQ; ++q) tap_acc_merge(&acc[q], in[q]);
   }
-  __shared__ tap_acc sh[kBlock];
-  block_reduce_acc(acc, kQ, sh);
+  __shared__ tap_acc sh[kBlock / kWave * kQ];
+  block_reduce_acc(acc, sh);
   if (threadIdx.x != 0) return;
   double gj[kJ];
   for (int j = 0; j < kJ; ++j) gj[j] = tap_acc_value(acc[3 + j]);
