@@ -1309,6 +1309,22 @@ __global__ void k_finish_step(VarState* st, int V, unsigned char* var_on) {
   var_on[v] = (unsigned char)st[v].active;
 }
 
+// Opens an FW iteration in one launch: the step flags, and the loading's
+// volumes, error flag and counters cleared.
+__global__ __launch_bounds__(kBlock) void k_iter_begin(VarState* st, int V, const unsigned char* var_on,
+                                                       int* err, unsigned long long* counters,
+                                                       double* y, long long ny) {
+  const long long i0 = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const long long stride = (long long)gridDim.x * kBlock;
+  for (long long v = i0; v < V; v += stride) {
+    st[v].stepping = var_on[v] ? 1 : 0;
+    st[v].need_more = 0;
+  }
+  if (i0 < 4) err[i0] = 0;
+  if (i0 < 16) counters[i0] = 0ull;
+  for (long long i = i0; i < ny; i += stride) y[i] = 0.0;
+}
+
 __global__ void k_begin_step(VarState* st, int V, const unsigned char* var_on) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= V) return;
@@ -1406,6 +1422,13 @@ struct tapgpu_ctx {
   bool defer_origin = false;
   bool origin_pending = false;
   int pending_Vn = 0;
+  // the volumes, error flag and counters of the next loading were cleared
+  // by k_iter_begin (the loading skips its own clears)
+  bool pre_cleared = false;
+  // error flag, counters, variant states and flags: one device block laid
+  // out like pin_block, read back with one copy
+  unsigned char* iter_dev = nullptr;
+  size_t iter_bytes = 0;
   // stats
   tapgpu_stats stats{};
   bool profiling = false;
@@ -1676,7 +1699,8 @@ int ensure_hbm(tapgpu_ctx* c);
 // workgroups (a few KB of LDS each) fill the rest.  Both add integer flows
 // to y with atomics, so the split changes no result.
 int launch_origin(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on, float* ms) {
-  HIPCHK(c, hipMemsetAsync(c->y + (size_t)v0 * c->E, 0, sizeof(double) * (size_t)Vn * c->E, c->stream));
+  if (!c->pre_cleared)
+    HIPCHK(c, hipMemsetAsync(c->y + (size_t)v0 * c->E, 0, sizeof(double) * (size_t)Vn * c->E, c->stream));
   if (c->Z > 0) {
     OriginArgs A = make_origin_args(c, var_on);
     A.fww += (size_t)v0 * c->E; A.rvw += (size_t)v0 * c->E; A.y += (size_t)v0 * c->E;
@@ -1767,13 +1791,10 @@ int absorb_origin(tapgpu_ctx* c, const int* herr, const unsigned long long* hc, 
                   std::chrono::steady_clock::time_point t0);
 
 // Reads the error flag and counters of an origin-kernel launch into stats:
-// now, or (defer_origin) into pinned memory, absorbed by absorb_pending
-// after the caller's next synchronisation.
+// now, or (defer_origin) from the pinned copy of the iteration block that
+// the caller makes, absorbed by absorb_pending after its synchronisation.
 int finish_origin(tapgpu_ctx* c, int Vn, std::chrono::steady_clock::time_point t0) {
-  if (c->defer_origin) {
-    HIPCHK(c, hipMemcpyAsync(c->pin_err, c->err_dev, sizeof(int) * 3, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->pin_cnt, c->counters, sizeof(unsigned long long) * 16,
-                             hipMemcpyDeviceToHost, c->stream));
+  if (c->defer_origin) {            // read back with the iteration's block copy
     c->origin_pending = true;
     c->pending_Vn = Vn;
     return TAPGPU_OK;
@@ -1830,7 +1851,7 @@ int absorb_origin(tapgpu_ctx* c, const int* herr, const unsigned long long* hc, 
 
 int aon_lds(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
   auto t0 = std::chrono::steady_clock::now();
-  HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
+  if (!c->pre_cleared) HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
   if (c->chase < 0 && c->Z > 0) {
     const int keep = c->split;
     c->split = 0;                       // chain length first, on the LDS kernel alone
@@ -1844,7 +1865,8 @@ int aon_lds(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
     if (rc) return rc;
     HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
   }
-  HIPCHK(c, hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 16, c->stream));
+  if (!c->pre_cleared)
+    HIPCHK(c, hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 16, c->stream));
   if (c->Z > 0) {
     if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     int rc = launch_origin(c, v0, Vn, var_on, nullptr);
@@ -1878,7 +1900,8 @@ constexpr double kDeltaCandidates[] = {0.0, 1.0, 2.0, 4.0, 8.0, 16.0};
 
 int launch_hbm(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on, int items_cap,
                float* ms) {
-  HIPCHK(c, hipMemsetAsync(c->y + (size_t)v0 * c->E, 0, sizeof(double) * (size_t)Vn * c->E, c->stream));
+  if (!c->pre_cleared)
+    HIPCHK(c, hipMemsetAsync(c->y + (size_t)v0 * c->E, 0, sizeof(double) * (size_t)Vn * c->E, c->stream));
   if (c->Z == 0) return TAPGPU_OK;
   OriginArgs A = make_origin_args(c, var_on);
   A.fww += (size_t)v0 * c->E; A.rvw += (size_t)v0 * c->E; A.y += (size_t)v0 * c->E;
@@ -1922,9 +1945,10 @@ int aon_hbm(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
   auto t0 = std::chrono::steady_clock::now();
   int rc = ensure_hbm(c);
   if (rc) return rc;
-  HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
+  if (!c->pre_cleared) HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
   if (c->delta_mult < 0 && c->Z > 0 && (rc = tune_delta(c, v0, Vn, var_on))) return rc;
-  HIPCHK(c, hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 16, c->stream));
+  if (!c->pre_cleared)
+    HIPCHK(c, hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 16, c->stream));
   if (c->profiling && c->Z > 0) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
   if ((rc = launch_hbm(c, v0, Vn, var_on, INT32_MAX, nullptr))) return rc;
   if (c->profiling && c->Z > 0) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
@@ -2192,7 +2216,7 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
   HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
   HIPCHK(c, hipHostMalloc((void**)&c->h_int, 64 * sizeof(int), hipHostMallocDefault));
   {
-    const size_t bytes = 16 + 128 + sizeof(VarState) * (size_t)V + (size_t)V;
+    const size_t bytes = 16 + 128 + sizeof(VarState) * (size_t)V + (size_t)V;   // as iter_dev
     HIPCHK(c, hipHostMalloc(&c->pin_block, bytes, hipHostMallocDefault));
     unsigned char* b = (unsigned char*)c->pin_block;
     c->pin_err = (int*)b;
@@ -2209,10 +2233,16 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
   ALLOC(col_src, c->P); ALLOC(grp_var, c->Galloc);
   c->flags_cap = (int)std::min<int64_t>(std::max<int64_t>(4 * N + 64, 1024), 1 << 24);
   ALLOC(flags, c->flags_cap);
-  ALLOC(counters, 16);
+  {
+    c->iter_bytes = 16 + 128 + sizeof(VarState) * (size_t)V + (size_t)V;
+    if ((rc = dalloc(c, &c->iter_dev, c->iter_bytes))) return rc;
+    c->err_dev = (int*)c->iter_dev;
+    c->counters = (unsigned long long*)(c->iter_dev + 16);
+    c->st = (VarState*)(c->iter_dev + 144);
+    c->var_on = c->iter_dev + 144 + sizeof(VarState) * (size_t)V;
+  }
 
-  ALLOC(err_dev, 4);
-  ALLOC(var_on, V); ALLOC(st, V);
+
   ALLOC(part, (size_t)V * kLinkBlocks * kQ);
 #undef ALLOC
   if ((rc = upload(c, c->fw_off, fw_off.data(), N + 1))) return rc;
@@ -2545,12 +2575,24 @@ int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_res
       if (maxit + 1 >= c->trace_cap && (rc = ensure_trace(c, 2 * c->trace_cap))) return rc;
     }
     auto t_it = std::chrono::steady_clock::now();
-    hipLaunchKernelGGL(k_begin_step, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->st, c->V,
-                       c->var_on);
+    // with the schedule already tuned, one kernel opens the iteration and
+    // clears what the loading would clear
+    const bool tuned = use_lds(c) ? (c->chase >= 0 && c->split >= 0)
+                                  : (use_hbm(c) && c->delta_mult >= 0);
+    if (tuned) {
+      const long long ny = (long long)c->V * c->E;
+      hipLaunchKernelGGL(k_iter_begin, dim3(grid_for(ny / kWave + 1)), dim3(kBlock), 0, c->stream, c->st,
+                         c->V, c->var_on, c->err_dev, c->counters, c->y, ny);
+    } else {
+      hipLaunchKernelGGL(k_begin_step, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->st, c->V,
+                         c->var_on);
+    }
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
     c->defer_origin = true;
+    c->pre_cleared = tuned;
     rc = load_variants(c, 0, c->V, c->var_on);
     c->defer_origin = false;
+    c->pre_cleared = false;
     if (rc) return rc;
     HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
     hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kBlock), 0, c->stream, K, c->st, c->part, 0);
@@ -2563,9 +2605,7 @@ int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_res
     hipLaunchKernelGGL(k_finish_step, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->st,
                        c->V, c->var_on);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(c->pin_st, c->st, sizeof(VarState) * c->V, hipMemcpyDeviceToHost,
-                             c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->pin_on, c->var_on, c->V, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->pin_block, c->iter_dev, c->iter_bytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if ((rc = absorb_pending(c))) return rc;
     c->stats.link_passes += 1;
