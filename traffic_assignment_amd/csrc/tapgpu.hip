@@ -473,6 +473,17 @@ struct OriginArgs {
 };
 
 __device__ __forceinline__ double as_d(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// wavefront-wide sum / max (every lane of the wavefront active), so that
+// per-thread counters reach a workgroup total with one LDS atomic per wave
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
+  for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+__device__ __forceinline__ int wave_max_i32(int x) {
+  for (int o = kWave / 2; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o));
+  return x;
+}
 __device__ __forceinline__ unsigned long long as_b(double d) { return (unsigned long long)__double_as_longlong(d); }
 
 // LDS layout (bytes): labels 8N (flows while loading) | three bitmaps
@@ -685,7 +696,10 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
   }
   if (!load) {
     if (A.counters) {
-      atomicAdd(&s_cnt[0], n_exp); atomicAdd(&s_cnt[1], n_scan); atomicAdd(&s_cnt[2], n_upd);
+      n_exp = wave_sum_u64(n_exp); n_scan = wave_sum_u64(n_scan); n_upd = wave_sum_u64(n_upd);
+      if (lane == 0) {
+        atomicAdd(&s_cnt[0], n_exp); atomicAdd(&s_cnt[1], n_scan); atomicAdd(&s_cnt[2], n_upd);
+      }
       __syncthreads();
       if (tid == 0) {
         atomicAdd(&A.counters[0], s_cnt[0]); atomicAdd(&A.counters[1], s_cnt[1]);
@@ -740,10 +754,16 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
     }
   }
   if (A.counters) {
-    atomicAdd(&s_cnt[0], n_exp); atomicAdd(&s_cnt[1], n_scan); atomicAdd(&s_cnt[2], n_upd);
-    atomicAdd(&s_steps, steps);
-    atomicMax(&s_maxlen, maxlen);
-    atomicAdd(&s_loaded, loaded);
+    n_exp = wave_sum_u64(n_exp); n_scan = wave_sum_u64(n_scan); n_upd = wave_sum_u64(n_upd);
+    steps = wave_sum_u64(steps);
+    maxlen = wave_max_i32(maxlen);
+    const unsigned long long ld = wave_sum_u64((unsigned long long)loaded);
+    if (lane == 0) {
+      atomicAdd(&s_cnt[0], n_exp); atomicAdd(&s_cnt[1], n_scan); atomicAdd(&s_cnt[2], n_upd);
+      atomicAdd(&s_steps, steps);
+      atomicMax(&s_maxlen, maxlen);
+      atomicAdd(&s_loaded, (int)ld);
+    }
   }
   __syncthreads();
   if (tid == 0 && A.counters) {
