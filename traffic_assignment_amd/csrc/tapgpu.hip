@@ -831,7 +831,7 @@ __device__ __forceinline__ int ldg_i32(const int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-__global__ __launch_bounds__(kHbmBlock) void k_origin_hbm(OriginArgs A, int load, int items) {
+__global__ __launch_bounds__(kHbmBlock, 4) void k_origin_hbm(OriginArgs A, int load, int items) {
   extern __shared__ unsigned long long sm[];
   const int N = A.N;
   const int W = (N + 63) >> 6;
@@ -921,14 +921,16 @@ __global__ __launch_bounds__(kHbmBlock) void k_origin_hbm(OriginArgs A, int load
         }
       }
     };
-    // u (listed with label lab) either stays queued (lab >= T) or expands
+    // u (listed with label lab) either stays queued (lab >= T) or expands;
+    // the smallest label kept queued by this lane in the round is min_kept
+    unsigned long long min_kept = ~0ull;
     auto visit = [&](int u, unsigned long long lab) {
       const int sl = r % 3;
 #if TAPGPU_HBM_FRESH
       if (!(as_d(lab) < T)) lab = ldg_u64(&dist[u]);
 #endif
       if (!(as_d(lab) < T)) {                 // later bucket: stays queued
-        atomicMin(&s_minq[sl], lab);
+        min_kept = min(min_kept, lab);
         ++n_def;
         list(true, u, lab);
         return;
@@ -992,6 +994,10 @@ __global__ __launch_bounds__(kHbmBlock) void k_origin_hbm(OriginArgs A, int load
           }
         }
       }
+      // one LDS atomic per wavefront for the round's smallest kept label
+      for (int o = kWave / 2; o > 0; o >>= 1) min_kept = min(min_kept, __shfl_xor(min_kept, o));
+      if (lane == 0 && min_kept != ~0ull) atomicMin(&s_minq[sl], min_kept);
+      min_kept = ~0ull;
       for (int w = tid; w < W; w += kHbmBlock) clr[w] = 0;
       if (tid == 0) {
         const int sn = (r + 1) % 3;
