@@ -433,9 +433,6 @@ constexpr int kLdsQueueMax = TAPGPU_LDS_QUEUE_MAX;   // frontier queue entries p
 #ifndef TAPGPU_LDS_GROUP
 #define TAPGPU_LDS_GROUP 6
 #endif
-#ifndef TAPGPU_LDS_PREREAD
-#define TAPGPU_LDS_PREREAD 0
-#endif
 constexpr int kGroup = TAPGPU_LDS_GROUP;             // lanes per expanded node (one link each)
 constexpr unsigned short kNoPred = 0xFFFF;
 
@@ -557,11 +554,8 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
   // it unless listed already; returns whether this lane lowered v
   auto relax = [&](double du, int v, double w, bool want) {
     const unsigned long long cb = as_b(du + w);
-#if TAPGPU_LDS_PREREAD
-    const bool low = cb < dist[v] && cb < atomicMin(&dist[v], cb);
-#else
+    // no pre-read of dist[v]: one LDS round trip, and measured faster
     const bool low = cb < atomicMin(&dist[v], cb);
-#endif
     bool first = false;
     if (low) {
       ++n_upd;
@@ -807,12 +801,6 @@ constexpr int kHbmBlock = TAPGPU_HBM_BLOCK;    // 512: 2 workgroups per CU, each
 #endif
 constexpr int kHbmPerCu = TAPGPU_HBM_PER_CU;    // workgroups sharing a CU's LDS (registers: <= 128 VGPRs)
 constexpr int kHbmQueueMax = 4096;              // frontier queue entries per round
-#ifndef TAPGPU_HBM_FRESH
-#define TAPGPU_HBM_FRESH 1
-#endif
-#ifndef TAPGPU_HBM_PREREAD
-#define TAPGPU_HBM_PREREAD 1
-#endif
 
 // LDS of the labels-in-HBM kernel: three bitmaps 3*8W | two queues of
 // (label 8 B, node 4 B) entries 2*12Q.
@@ -926,9 +914,7 @@ __global__ __launch_bounds__(kHbmBlock, 4) void k_origin_hbm(OriginArgs A, int l
     unsigned long long min_kept = ~0ull;
     auto visit = [&](int u, unsigned long long lab) {
       const int sl = r % 3;
-#if TAPGPU_HBM_FRESH
-      if (!(as_d(lab) < T)) lab = ldg_u64(&dist[u]);
-#endif
+      if (!(as_d(lab) < T)) lab = ldg_u64(&dist[u]);   // the listed label may be stale
       if (!(as_d(lab) < T)) {                 // later bucket: stays queued
         min_kept = min(min_kept, lab);
         ++n_def;
@@ -947,22 +933,18 @@ __global__ __launch_bounds__(kHbmBlock, 4) void k_origin_hbm(OriginArgs A, int l
 #pragma unroll
         for (int k = 0; k < 8; ++k)
           if (e0 + k < ee) { hv[k] = A.fw_dst[e0 + k]; hw[k] = fw[e0 + k]; }
-#if TAPGPU_HBM_PREREAD
+        // the heads' labels are read first: most links lower nothing, and an
+        // L2 atomic costs more than a read
 #pragma unroll
         for (int k = 0; k < 8; ++k)
           if (e0 + k < ee) hd[k] = ldg_u64(&dist[hv[k]]);
-#endif
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
           if (e0 + k >= ee) break;
           const int v = hv[k];
           const unsigned long long cb = as_b(du + hw[k]);
           bool low = false;
-#if TAPGPU_HBM_PREREAD
           if (cb < hd[k]) {
-#else
-          {
-#endif
             const unsigned long long old =
                 __hip_atomic_fetch_min(&dist[v], cb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (cb < old) {
