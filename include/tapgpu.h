@@ -226,12 +226,13 @@ int tapgpu_set_strategy(tapgpu_ctx *ctx, int strategy);
 int tapgpu_set_origin_chase(tapgpu_ctx *ctx, int chase);
 int tapgpu_get_origin_chase(const tapgpu_ctx *ctx);
 
-/* Frontier queue of the one-origin-per-workgroup path (strategy 2): a round
-   whose frontier (the nodes lowered in the round before) has at most this
-   many nodes expands them from an LDS list, eight lanes per node; a larger
-   frontier is found by scanning its bitmap.  -1 (the default): the most
-   the workgroup's spare LDS holds (up to 4096); 0: bitmap scans only;
-   larger values are clamped to that most.  Results do not depend on it. */
+/* Frontier queues of the one-origin-per-workgroup paths (strategies 2 and
+   3): a round whose frontier (the nodes lowered or kept in the round
+   before) has at most this many nodes takes them from an LDS list; a
+   larger frontier is found by scanning its bitmap.  -1 (the default): the
+   most each kernel's spare LDS holds (up to 4096); 0: bitmap scans only;
+   larger values are clamped to that most.  tapgpu_get_origin_queue returns
+   the value of the kernel in use.  Results do not depend on it. */
 int tapgpu_set_origin_queue(tapgpu_ctx *ctx, int entries);
 int tapgpu_get_origin_queue(const tapgpu_ctx *ctx);
 

@@ -302,6 +302,12 @@ def test_schedule_knobs_do_not_change_results(engine_mod):
         for d in (0.0, 0.5, 4.0, 64.0):
             eng.set_origin_delta(d)
             out.append(eng.aon())
+        for q in (0, 16):                     # bitmap scans only; mixed rounds
+            eng.set_origin_queue(q)
+            out.append(eng.aon())
+        assert eng.origin_queue == 16
+        eng.set_origin_queue(-1)
+        assert eng.origin_queue >= 64
         with pytest.raises(engine_mod.TapGpuError):
             eng.set_origin_delta(float("nan"))
         eng.set_origin_delta(-1)          # tune again on the next loading

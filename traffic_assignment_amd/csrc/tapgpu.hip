@@ -1397,7 +1397,8 @@ struct tapgpu_ctx {
   size_t lds_bytes = 0;
   int qcap = 0;                       // LDS frontier queue entries in use (per buffer)
   int qcap_max = 0;                   // entries the LDS allocation holds
-  int hqcap = 0;                      // labels-in-HBM kernel: queue entries (per buffer)
+  int hqcap = 0;                      // labels-in-HBM kernel: queue entries in use (per buffer)
+  int hqcap_max = 0;                  // entries its LDS allocation holds
   // labels
   double* dist = nullptr;
   int *pred = nullptr, *pend = nullptr, *col_src = nullptr, *grp_var = nullptr;
@@ -2171,8 +2172,9 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
         lds_cu = lds_max;
       const size_t share = std::min<size_t>((size_t)lds_cu / kHbmPerCu, (size_t)lds_max) - 256;
       const size_t spare = share > c->hbm_lds ? share - c->hbm_lds : 0;
-      c->hqcap = (int)std::min<size_t>(kHbmQueueMax, spare / 24) & ~63;
-      c->hbm_lds = origin_hbm_lds_bytes(N, c->hqcap);
+      c->hqcap_max = (int)std::min<size_t>(kHbmQueueMax, spare / 24) & ~63;
+      c->hqcap = c->hqcap_max;
+      c->hbm_lds = origin_hbm_lds_bytes(N, c->hqcap_max);
     }
     int ncu = 0, per_cu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) ncu = 1;
@@ -2797,12 +2799,14 @@ int tapgpu_get_origin_chase(const tapgpu_ctx* c) {
 int tapgpu_set_origin_queue(tapgpu_ctx* c, int entries) {
   if (!c || entries < -1) return TAPGPU_E_ARG;
   c->qcap = entries < 0 ? c->qcap_max : std::min(entries, c->qcap_max);
+  c->hqcap = entries < 0 ? c->hqcap_max : std::min(entries, c->hqcap_max);
   return TAPGPU_OK;
 }
 
 int tapgpu_get_origin_queue(const tapgpu_ctx* c) {
   if (!c) return TAPGPU_E_ARG;
-  return c->qcap;
+  tapgpu_ctx* m = const_cast<tapgpu_ctx*>(c);
+  return use_hbm(m) ? c->hqcap : c->qcap;
 }
 
 int tapgpu_get_strategy(const tapgpu_ctx* c) {

@@ -306,8 +306,9 @@ class TapEngine:
         return lib().tapgpu_get_origin_chase(self._h)
 
     def set_origin_queue(self, entries):
-        """Largest frontier the one-origin-per-workgroup path expands from
-        its LDS queue (-1: as many as its LDS holds, 0: bitmap scans only)."""
+        """Largest frontier the one-origin-per-workgroup paths take from
+        their LDS queues (-1: as many as their LDS holds, 0: bitmap scans
+        only)."""
         self._check(lib().tapgpu_set_origin_queue(self._h, int(entries)))
 
     @property
