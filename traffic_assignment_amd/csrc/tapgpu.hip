@@ -415,12 +415,18 @@ __global__ __launch_bounds__(kBlock) void k_aon_reduce(int E, int Gv, int v0, in
 // covers every (variant, origin).
 //
 // Predecessor rule: among in-links u->v with fl(dist[u] + w) == dist[v]
-// (u expandable), take the tail with the smallest label (the one a
-// label-ordered search such as d'Esopo-Pape-LLL normally expands first),
-// then the smallest (round in which its label was last lowered, node id).
-// Along the chosen links (label, key) strictly decreases, so the tree is
-// acyclic even through zero-cost links; where the shortest path is unique
-// this is the reference's predecessor.
+// (u expandable) and dist[u] < dist[v], take the tail with the smallest
+// label (the one a label-ordered search such as d'Esopo-Pape-LLL normally
+// expands first), then the smallest (round in which its label was last
+// lowered, node id).  A node whose tight tails all carry its own label
+// (zero-cost links, or costs below half an ulp of the label) has none; such
+// nodes join the tree level by level (attach_level_ties): each pass gives
+// every one of them the smallest-id tight tail that was in the tree before
+// the pass.  Along the chosen links the label decreases, or stays equal
+// while the level decreases, so the tree is acyclic.  (Rounds alone do not
+// order equal labels: a node lowered during a round can be expanded in the
+// same round with its new label.)  Where the shortest path is unique this
+// is the reference's predecessor.
 
 constexpr int kLdsBlock = 1024;
 constexpr int kDefaultChase = 4;               // before tuning / shortest-path-only calls
@@ -489,6 +495,52 @@ __device__ __forceinline__ unsigned long long as_b(double d) { return (unsigned 
 size_t origin_lds_bytes(int N, int qcap) {
   const size_t W = (size_t)(N + 63) / 64, A = (size_t)((N + 1 + 3) & ~3);
   return 8ull * N + 24ull * W + 4ull * A + 4ull * (size_t)qcap;
+}
+
+// Second step of the predecessor rule: the nodes flagged in `pending` have
+// no tight tail with a smaller label.  Each pass gives every pending node
+// the smallest-id tight tail with its own label that was in the tree before
+// the pass (not pending), and clears the flags after a barrier, so a pass's
+// result does not depend on thread timing.  Every pending node hangs from a
+// tree node through such links (its label came along them), so each pass
+// attaches at least one node; false if a pass attaches none.  All threads
+// of the workgroup call it; label(v) reads a label, set_pred(v, link, tail)
+// records a choice.
+template <int kThreads, class Label, class SetPred>
+__device__ bool attach_level_ties(const OriginArgs& A, const double* __restrict__ rw, int src,
+                                  unsigned long long* pending, unsigned long long* added,
+                                  int* s_left, int* s_added, Label label, SetPred set_pred) {
+  const int N = A.N, W = (N + 63) >> 6, tid = threadIdx.x;
+  for (int w = tid; w < W; w += kThreads) added[w] = 0;
+  for (;;) {
+    if (tid == 0) { *s_left = 0; *s_added = 0; }
+    __syncthreads();
+    for (int v = tid; v < N; v += kThreads) {
+      if (!((pending[v >> 6] >> (v & 63)) & 1ull)) continue;
+      const unsigned long long dv = label(v);
+      int bu = -1, be = -1;
+      for (int k = A.rv_off[v]; k < A.rv_off[v + 1]; ++k) {
+        const int u = A.rv_src[k];
+        if (u == v || (u < A.ft0 && u != src)) continue;
+        if ((pending[u >> 6] >> (u & 63)) & 1ull) continue;       // not in the tree yet
+        const unsigned long long du = label(u);
+        if (du != dv || as_b(as_d(du) + rw[k]) != dv) continue;
+        if (bu < 0 || u < bu) { bu = u; be = A.rv_eid[k]; }
+      }
+      if (be >= 0) {
+        set_pred(v, be, bu);
+        atomicOr(&added[v >> 6], 1ull << (v & 63));
+        atomicAdd(s_added, 1);
+      } else {
+        atomicAdd(s_left, 1);
+      }
+    }
+    __syncthreads();
+    const int left = *s_left, got = *s_added;
+    for (int w = tid; w < W; w += kThreads) { pending[w] &= ~added[w]; added[w] = 0; }
+    if (left == 0 || got == 0) return left == 0;
+    __syncthreads();
+  }
 }
 
 __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load) {
@@ -637,7 +689,10 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
   //      each node's tails, costs and link ids are loaded together, so a
   //      node costs one L2 round trip.  Results stay in registers until
   //      every thread is done reading offsets and rounds. ----
+  __shared__ int s_level_ties;
   for (int v = tid; v <= N; v += kLdsBlock) offs[v] = (unsigned short)A.rv_off[v];
+  for (int w = tid; w < W; w += kLdsBlock) bmB[w] = 0;     // nodes left for attach_level_ties
+  if (tid == 0) s_level_ties = 0;
   __syncthreads();
   unsigned short my_tail[kLdsPerThread], my_pred[kLdsPerThread];
   {
@@ -646,8 +701,9 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
       unsigned short p = kNoPred, pt = 0;
       const unsigned long long dv = dist[v];
       if (v != src && dv != kMax) {
-        unsigned best = 0xFFFFFFFFu;
-        unsigned long long best_du = ~0ull;
+        // starting from (dv, 0) only tails with a smaller label can win
+        unsigned best = 0;
+        unsigned long long best_du = dv;
         const int kb = offs[v], ke = offs[v + 1];
         for (int k0 = kb; k0 < ke; k0 += 8) {
           int hu[8], he[8];
@@ -675,10 +731,17 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
       }
       my_tail[j] = pt;
       my_pred[j] = p;
+      // reached, not the origin, and no tight tail with a smaller label:
+      // every tight tail has this label
+      const bool level_tie = p == kNoPred && v != src && dv != kMax;
+      if (level_tie) {
+        atomicOr(&bmB[v >> 6], 1ull << (v & 63));
+        s_level_ties = 1;
+      }
       if (A.out_dist) {
         const size_t o = (size_t)v * A.ld + blockIdx.x;
         A.out_dist[o] = as_d(dv);
-        A.out_pred[o] = (p == kNoPred) ? -1 : (int)p;
+        if (!level_tie) A.out_pred[o] = (p == kNoPred) ? -1 : (int)p;
       }
     }
   }
@@ -687,6 +750,17 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
   {
     int j = 0;
     for (int v = tid; v < N; v += kLdsBlock, ++j) { pred[v] = my_pred[j]; ptail[v] = my_tail[j]; }
+  }
+  if (s_level_ties) {
+    __shared__ int s_left, s_added;
+    const bool ok = attach_level_ties<kLdsBlock>(
+        A, rw, src, bmB, bmC, &s_left, &s_added, [&](int v) { return dist[v]; },
+        [&](int v, int e, int u) {
+          pred[v] = (unsigned short)e;
+          ptail[v] = (unsigned short)u;
+          if (A.out_dist) A.out_pred[(size_t)v * A.ld + blockIdx.x] = e;
+        });
+    if (!ok && tid == 0 && atomicCAS(&A.err[0], 0, 3) == 0) { A.err[1] = src; A.err[2] = 0; }
   }
   if (!load) {
     if (A.counters) {
@@ -835,6 +909,7 @@ __global__ __launch_bounds__(kHbmBlock, 4) void k_origin_hbm(OriginArgs A, int l
   __shared__ unsigned long long s_minq[3];
   __shared__ unsigned long long s_cnt[8];
   __shared__ int s_maxlen;
+  __shared__ int s_level_ties, s_left, s_added;
   const int tid = threadIdx.x;
   const unsigned long long kMax = as_b(DBL_MAX);
   unsigned long long* dist = A.h_dist + (size_t)blockIdx.x * N;
@@ -999,15 +1074,19 @@ __global__ __launch_bounds__(kHbmBlock, 4) void k_origin_hbm(OriginArgs A, int l
     n_rounds += (unsigned long long)r;
     const long long clk1 = wall_clock64();
 
-    // ---- predecessor links: smallest tail label, then (round, node).
+    // ---- predecessor links: smallest tail label below the node's, then
+    //      (round, node); nodes with none join by attach_level_ties.
     //      Tails, costs and link ids of up to 4 in-links are loaded
     //      together, then the tails' labels; rounds are read only to break
     //      an exact tie of labels. ----
+    for (int w = tid; w < W; w += kHbmBlock) bmB[w] = 0;     // nodes left for attach_level_ties
+    if (tid == 0) s_level_ties = 0;
+    __syncthreads();
     for (int v = tid; v < N; v += kHbmBlock) {
       int p = -1, pt = -1;
       const unsigned long long dv = ldg_u64(&dist[v]);
       if (v != src && dv != kMax) {
-        unsigned long long best_du = ~0ull;
+        unsigned long long best_du = dv;      // only tails with a smaller label can win
         const int kb = A.rv_off[v], ke = A.rv_off[v + 1];
         for (int k0 = kb; k0 < ke; k0 += 4) {
           int hu[4], he[4];
@@ -1028,7 +1107,7 @@ __global__ __launch_bounds__(kHbmBlock, 4) void k_origin_hbm(OriginArgs A, int l
             if (du == kMax) continue;
             if (as_b(as_d(du) + hw[i]) != dv) continue;
             bool take = du < best_du;
-            if (!take && du == best_du) {
+            if (!take && du == best_du && pt >= 0) {
               const unsigned long long key = ((unsigned long long)ldg_u32(&rnd[u]) << 32) | (unsigned)u;
               const unsigned long long bkey = ((unsigned long long)ldg_u32(&rnd[pt]) << 32) | (unsigned)pt;
               take = key < bkey;
@@ -1043,13 +1122,31 @@ __global__ __launch_bounds__(kHbmBlock, 4) void k_origin_hbm(OriginArgs A, int l
       }
       __hip_atomic_store(&pred[v], p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       __hip_atomic_store(&ptail[v], pt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      // reached, not the origin, and no tight tail with a smaller label
+      const bool level_tie = p < 0 && v != src && dv != kMax;
+      if (level_tie) {
+        atomicOr(&bmB[v >> 6], 1ull << (v & 63));
+        s_level_ties = 1;
+      }
       if (A.out_dist) {
         const size_t o = (size_t)v * A.ld + item;
         A.out_dist[o] = as_d(dv);
-        A.out_pred[o] = p;
+        if (!level_tie) A.out_pred[o] = p;
       }
     }
-    __syncthreads();          // every label read is done: the row now holds flows
+    __syncthreads();
+    if (s_level_ties) {
+      const bool ok = attach_level_ties<kHbmBlock>(
+          A, rw, src, bmB, bmC, &s_left, &s_added, [&](int v) { return ldg_u64(&dist[v]); },
+          [&](int v, int e, int u) {
+            __hip_atomic_store(&pred[v], e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(&ptail[v], u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (A.out_dist) A.out_pred[(size_t)v * A.ld + item] = e;
+          });
+      if (!ok && tid == 0 && atomicCAS(&A.err[0], 0, 3) == 0) { A.err[1] = src; A.err[2] = 0; }
+      __syncthreads();
+    }
+    // every label read is done: the row now holds flows
     const long long clk2 = wall_clock64();
     c_relax += (unsigned long long)(clk1 - clk0);
     c_pred += (unsigned long long)(clk2 - clk1);
