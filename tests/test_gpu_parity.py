@@ -110,12 +110,24 @@ def random_network(seed, cost_kind):
         cost = 10.0 ** rng.uniform(-6, 6, E)
     else:
         cost = rng.uniform(0.01, 5.0, E)
-        cost[rng.random(E) < 0.15] = 0.0
+        if cost_kind == "cont":
+            cost[rng.random(E) < 0.15] = 0.0
     net = synth.Network(num_zones=Z, num_nodes=N, first_thru_node=ft, init=init, term=term,
-                        capacity=np.ones(E), length=np.ones(E), fftime=cost, B=np.zeros(E),
-                        power=np.ones(E), speed=np.zeros(E), toll=np.zeros(E),
-                        ltype=np.ones(E, np.int16))
+                        capacity=rng.uniform(20.0, 200.0, E), length=np.ones(E), fftime=cost,
+                        B=np.full(E, 0.15), power=np.full(E, 4.0), speed=np.zeros(E),
+                        toll=np.zeros(E), ltype=np.ones(E, np.int16))
     return net, rng
+
+
+def random_demand(net, rng):
+    """Integer demand from every zone to a random subset of the others."""
+    from traffic_assignment_amd import synth
+    Z = net.num_zones
+    dest = [np.sort(rng.choice(np.setdiff1d(np.arange(1, Z + 1), [o]),
+                               size=int(rng.integers(0, Z)), replace=False)) for o in range(1, Z + 1)]
+    start = np.r_[0, np.cumsum([d.size for d in dest])].astype(np.int64)
+    allr = np.concatenate(dest).astype(np.int64)
+    return synth.Demand(Z, start, allr, rng.integers(0, 30, allr.size).astype(np.float64))
 
 
 def pred_reaches_origin(net, dist, pred, origins):
@@ -162,18 +174,35 @@ def test_sssp_random_networks(engine_mod, oracle_lib, seed, cost_kind, strategy)
 def test_aon_random_networks(engine_mod, oracle_lib, seed, cost_kind, strategy):
     """Integer demand between random zone pairs on random networks: volumes
     exactly the oracle's."""
-    from traffic_assignment_amd import synth
     print(f"case aon random seed={seed} {cost_kind} {strategy}", flush=True)
     net, rng = random_network(seed, cost_kind)
-    Z = net.num_zones
-    dest = [np.sort(rng.choice(np.setdiff1d(np.arange(1, Z + 1), [o]),
-                               size=int(rng.integers(0, Z)), replace=False)) for o in range(1, Z + 1)]
-    start = np.r_[0, np.cumsum([d.size for d in dest])].astype(np.int64)
-    allr = np.concatenate(dest).astype(np.int64)
-    dem = synth.Demand(Z, start, allr, rng.integers(0, 30, allr.size).astype(np.float64))
+    dem = random_demand(net, rng)
     with make_engine(engine_mod, net, dem, strategy) as eng:
         y = eng.aon()
     assert np.array_equal(y, oracle_lib.fw_assign(net, dem, net.fftime, nthreads=8))
+
+
+@pytest.mark.parametrize("strategy", STRATS)
+@pytest.mark.parametrize("seed", [10, 11])
+def test_fw_random_networks(engine_mod, oracle_lib, seed, strategy):
+    """30 FW iterations on random networks (BPR, power 4, random
+    capacities, positive free-flow times): step sizes, volumes and costs
+    bit-identical to the oracle with the correctly rounded power."""
+    print(f"case fw random seed={seed} {strategy}", flush=True)
+    net, rng = random_network(seed, "pos")
+    dem = random_demand(net, rng)
+    with make_engine(engine_mod, net, dem, strategy) as eng:
+        eng.fw_init()
+        eng.fw_run(max_iters=30)
+        x, c = eng.flows()
+        tr = eng.trace()
+    oracle_lib.lib().ora_set_pow_mode(1)
+    try:
+        o = oracle_lib.fw_solve(net, dem, max_iters=30, nthreads=8)
+    finally:
+        oracle_lib.lib().ora_set_pow_mode(0)
+    assert np.array_equal(tr["lambda_"], o["trace"]["lambda_"])
+    assert np.array_equal(x, o["x"]) and np.array_equal(c, o["c"])
 
 
 def test_sssp_pape_cuda_layout(engine_mod, oracle_lib):
