@@ -111,7 +111,8 @@ int tapgpu_set_costs(tapgpu_ctx *ctx, int64_t variant, const double *costs);
    unreachable; pred is
    the link index of the last link of the path (-1: none).  Distances are
    bit-identical to the reference's CPU d'Esopo-Pape; pred matches wherever
-   the shortest path is unique. */
+   the shortest path is unique, and the pred links of each start node form a
+   tree rooted at it, zero-cost links included. */
 int tapgpu_sssp(tapgpu_ctx *ctx, int64_t variant, const int64_t *start_nodes,
                 int64_t num_start, double *dist, int64_t *pred, int64_t ld,
                 int64_t col_base);
