@@ -71,3 +71,30 @@ def test_greedy_cli_matches_reference_output(engine_mod):
     assert "initial unsatifised demand = " in err
     np.testing.assert_allclose(v, b64_array(ref["x_b64"], np.float64), rtol=0, atol=1e-9)
     np.testing.assert_allclose(c, b64_array(ref["c_b64"], np.float64), rtol=1e-14)
+
+
+@pytest.mark.parametrize("strategy", ["sweeps", "lds", "hbm"])
+@pytest.mark.parametrize("seed", [10, 11])
+def test_greedy_random_networks(engine_mod, oracle_lib, seed, strategy):
+    """Greedy assignment on the random networks of the parity tests
+    (parallel links, self-loops, non-through zones): every iteration's
+    unsatisfied demand, the volumes and the costs bit-identical to the
+    oracle."""
+    from test_gpu_parity import random_demand, random_network
+    print(f"case greedy random seed={seed} {strategy}", flush=True)
+    net, rng = random_network(seed, "pos")
+    dem = random_demand(net, rng)
+    with engine_mod.TapEngine(net, dem) as eng:
+        if strategy == "lds" and eng.strategy != "lds":
+            pytest.skip("network does not fit the LDS path")
+        eng.set_strategy(strategy)
+        g = eng.greedy()
+    oracle_lib.lib().ora_set_pow_mode(1)
+    try:
+        o = oracle_lib.greedy(net, dem)
+    finally:
+        oracle_lib.lib().ora_set_pow_mode(0)
+    assert g["iterations"] == o["iterations"] and g["stalled"] == o["stalled"]
+    assert np.array_equal(g["unsat_trace"], o["unsat_trace"])
+    assert np.array_equal(g["x"], o["x"]) and np.array_equal(g["c"], o["c"])
+    assert g["total_cost"] == o["total_cost"]
