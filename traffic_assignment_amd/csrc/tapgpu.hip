@@ -417,16 +417,15 @@ __global__ __launch_bounds__(kBlock) void k_aon_reduce(int E, int Gv, int v0, in
 // Predecessor rule: among in-links u->v with fl(dist[u] + w) == dist[v]
 // (u expandable) and dist[u] < dist[v], take the tail with the smallest
 // label (the one a label-ordered search such as d'Esopo-Pape-LLL normally
-// expands first), then the smallest (round in which its label was last
-// lowered, node id).  A node whose tight tails all carry its own label
+// expands first), then the smallest tail id, then the first such link in
+// link order -- the choice of the node-major sweeps too, and independent of
+// thread timing.  A node whose tight tails all carry its own label
 // (zero-cost links, or costs below half an ulp of the label) has none; such
 // nodes join the tree level by level (attach_level_ties): each pass gives
 // every one of them the smallest-id tight tail that was in the tree before
 // the pass.  Along the chosen links the label decreases, or stays equal
-// while the level decreases, so the tree is acyclic.  (Rounds alone do not
-// order equal labels: a node lowered during a round can be expanded in the
-// same round with its new label.)  Where the shortest path is unique this
-// is the reference's predecessor.
+// while the level decreases, so the tree is acyclic.  Where the shortest
+// path is unique this is the reference's predecessor.
 
 constexpr int kLdsBlock = 1024;
 constexpr int kDefaultChase = 4;               // before tuning / shortest-path-only calls
@@ -470,7 +469,6 @@ struct OriginArgs {
   long long ld;
   // labels-in-HBM kernel: per-block scratch rows of N entries
   unsigned long long* h_dist;        // labels, then flows
-  unsigned* h_rnd;                   // round of the last lowering
   int* h_pred;                       // tree link
   int* h_ptail;                      // its tail
 };
@@ -490,7 +488,7 @@ __device__ __forceinline__ int wave_max_i32(int x) {
 __device__ __forceinline__ unsigned long long as_b(double d) { return (unsigned long long)__double_as_longlong(d); }
 
 // LDS layout (bytes): labels 8N (flows while loading) | three bitmaps
-// 3*8W | offs/pred 2A | rnd/ptail 2A | two frontier queues 2*2Q, with
+// 3*8W | offs/pred 2A | ptail 2A | two frontier queues 2*2Q, with
 // W = ceil(N/64), A = N+1 rounded up to 4, Q = queue entries.
 size_t origin_lds_bytes(int N, int qcap) {
   const size_t W = (size_t)(N + 63) / 64, A = (size_t)((N + 1 + 3) & ~3);
@@ -554,9 +552,8 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
   unsigned long long* bmC = bmB + W;
   unsigned short* offs = (unsigned short*)(bmC + W);               // [N+1], then pred [N]
   unsigned short* pred = offs;
-  unsigned short* rnd = offs + Al;                                 // [N], then ptail [N]
-  unsigned short* ptail = rnd;
-  unsigned short* queue0 = rnd + Al;                               // 2 x [qcap]
+  unsigned short* ptail = offs + Al;                               // [N]
+  unsigned short* queue0 = ptail + Al;                             // 2 x [qcap]
   unsigned short* queue1 = queue0 + A.qcap;
   auto qw_of = [&](int rr) { return (rr & 1) ? queue1 : queue0; };   // queue written in round rr
   __shared__ int s_qn[3];
@@ -570,7 +567,7 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
   const double* __restrict__ rw = A.rvw + (size_t)var * A.E;
   const unsigned long long kMax = as_b(DBL_MAX);
 
-  for (int v = tid; v < N; v += kLdsBlock) { dist[v] = kMax; rnd[v] = 0; }
+  for (int v = tid; v < N; v += kLdsBlock) dist[v] = kMax;
   for (int v = tid; v <= N; v += kLdsBlock) offs[v] = (unsigned short)A.fw_off[v];
   for (int w = tid; w < 3 * W; w += kLdsBlock) bmA[w] = 0;
   if (tid < 3) s_qn[tid] = tid == 0 ? 1 : 0;
@@ -611,7 +608,6 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
     bool first = false;
     if (low) {
       ++n_upd;
-      rnd[v] = (unsigned short)r;
       if (want && v >= A.ft0) {
         const unsigned long long bit = 1ull << (v & 63);
         first = !(atomicOr(&nxt[v >> 6], bit) & bit);
@@ -684,11 +680,11 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
   const int relax_rounds = r;
   const long long clk1 = wall_clock64();
 
-  // ---- predecessor links (label, then round/node key; see above).  The
-  //      in-link offsets are staged in LDS (the out-link offsets are dead);
-  //      each node's tails, costs and link ids are loaded together, so a
-  //      node costs one L2 round trip.  Results stay in registers until
-  //      every thread is done reading offsets and rounds. ----
+  // ---- predecessor links (label, then tail id; see above).  The in-link
+  //      offsets are staged in LDS (the out-link offsets are dead); each
+  //      node's tails, costs and link ids are loaded together, so a node
+  //      costs one L2 round trip.  Results stay in registers until every
+  //      thread is done reading offsets (pred shares their space). ----
   __shared__ int s_level_ties;
   for (int v = tid; v <= N; v += kLdsBlock) offs[v] = (unsigned short)A.rv_off[v];
   for (int w = tid; w < W; w += kLdsBlock) bmB[w] = 0;     // nodes left for attach_level_ties
@@ -719,7 +715,7 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
             const unsigned long long du = dist[u];
             if (du == kMax) continue;
             if (as_b(as_d(du) + hw[i]) != dv) continue;
-            const unsigned key = ((unsigned)rnd[u] << 16) | (unsigned)u;
+            const unsigned key = (unsigned)u;
             if (du < best_du || (du == best_du && key < best)) {
               best = key;
               best_du = du;
@@ -745,7 +741,7 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
       }
     }
   }
-  __syncthreads();        // every offs[] and rnd[] read is done
+  __syncthreads();        // every offs[] read is done
   const long long clk2 = wall_clock64();
   {
     int j = 0;
@@ -854,7 +850,7 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
 //
 // The same three phases as k_origin_lds for networks whose labels do not fit
 // LDS: only the three frontier bitmaps (3 x 8 x ceil(N/64) bytes) live in
-// LDS; labels, rounds, predecessors and flows are a per-workgroup scratch
+// LDS; labels, predecessors and flows are a per-workgroup scratch
 // row in HBM (L2 / infinity cache resident while the workgroup runs).  A
 // persistent grid loops over (origin, variant) items.  A row is private to
 // its workgroup, whose waves share one CU, so every scratch access is a
@@ -913,7 +909,6 @@ __global__ __launch_bounds__(kHbmBlock, 4) void k_origin_hbm(OriginArgs A, int l
   const int tid = threadIdx.x;
   const unsigned long long kMax = as_b(DBL_MAX);
   unsigned long long* dist = A.h_dist + (size_t)blockIdx.x * N;
-  unsigned* rnd = A.h_rnd + (size_t)blockIdx.x * N;
   int* pred = A.h_pred + (size_t)blockIdx.x * N;
   int* ptail = A.h_ptail + (size_t)blockIdx.x * N;
   double* flow = (double*)dist;
@@ -932,7 +927,6 @@ __global__ __launch_bounds__(kHbmBlock, 4) void k_origin_hbm(OriginArgs A, int l
     const long long clk0 = wall_clock64();
     for (int v = tid; v < N; v += kHbmBlock) {
       __hip_atomic_store(&dist[v], kMax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      __hip_atomic_store(&rnd[v], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     for (int w = tid; w < 3 * W; w += kHbmBlock) bmA[w] = 0;
     if (tid < 3) { s_qn[tid] = tid == 0 ? 1 : 0; s_exp[tid] = 0; s_minq[tid] = ~0ull; }
@@ -1024,7 +1018,6 @@ __global__ __launch_bounds__(kHbmBlock, 4) void k_origin_hbm(OriginArgs A, int l
                 __hip_atomic_fetch_min(&dist[v], cb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (cb < old) {
               ++n_upd;
-              __hip_atomic_store(&rnd[v], (unsigned)r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
               low = v >= A.ft0;
             }
           }
@@ -1075,10 +1068,9 @@ __global__ __launch_bounds__(kHbmBlock, 4) void k_origin_hbm(OriginArgs A, int l
     const long long clk1 = wall_clock64();
 
     // ---- predecessor links: smallest tail label below the node's, then
-    //      (round, node); nodes with none join by attach_level_ties.
-    //      Tails, costs and link ids of up to 4 in-links are loaded
-    //      together, then the tails' labels; rounds are read only to break
-    //      an exact tie of labels. ----
+    //      tail id; nodes with none join by attach_level_ties.  Tails,
+    //      costs and link ids of up to 4 in-links are loaded together, then
+    //      the tails' labels. ----
     for (int w = tid; w < W; w += kHbmBlock) bmB[w] = 0;     // nodes left for attach_level_ties
     if (tid == 0) s_level_ties = 0;
     __syncthreads();
@@ -1107,11 +1099,7 @@ __global__ __launch_bounds__(kHbmBlock, 4) void k_origin_hbm(OriginArgs A, int l
             if (du == kMax) continue;
             if (as_b(as_d(du) + hw[i]) != dv) continue;
             bool take = du < best_du;
-            if (!take && du == best_du && pt >= 0) {
-              const unsigned long long key = ((unsigned long long)ldg_u32(&rnd[u]) << 32) | (unsigned)u;
-              const unsigned long long bkey = ((unsigned long long)ldg_u32(&rnd[pt]) << 32) | (unsigned)pt;
-              take = key < bkey;
-            }
+            if (!take && du == best_du && pt >= 0) take = u < pt;
             if (take) {
               best_du = du;
               p = he[i];
@@ -1481,7 +1469,6 @@ struct tapgpu_ctx {
   double delta_mult = -1;             // bucket width / mean free-flow time (-1: tune)
   double mean_t0 = 1;                 // mean free-flow time of variant 0's links
   unsigned long long* h_dist = nullptr;
-  unsigned* h_rnd = nullptr;
   int *h_pred = nullptr, *h_ptail = nullptr;
   int clock_khz = 0;                  // device wall clock
   bool labels_ready = false;          // node-major buffers allocated
@@ -1780,7 +1767,7 @@ OriginArgs make_origin_args(tapgpu_ctx* c, const unsigned char* var_on) {
   A.chase = c->chase >= 0 ? c->chase : kDefaultChase;
   A.qcap = c->qcap;
   A.hqcap = c->hqcap;
-  A.h_dist = c->h_dist; A.h_rnd = c->h_rnd; A.h_pred = c->h_pred; A.h_ptail = c->h_ptail;
+  A.h_dist = c->h_dist; A.h_pred = c->h_pred; A.h_ptail = c->h_ptail;
   A.delta = c->delta_mult > 0 ? c->delta_mult * c->mean_t0 : 0.0;
   A.z0 = 0;
   A.zn = c->Z;
@@ -1816,7 +1803,7 @@ int launch_origin(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on, fl
     if (zh > 0) {
       int rc = ensure_hbm(c);
       if (rc) return rc;
-      A.h_dist = c->h_dist; A.h_rnd = c->h_rnd; A.h_pred = c->h_pred; A.h_ptail = c->h_ptail;
+      A.h_dist = c->h_dist; A.h_pred = c->h_pred; A.h_ptail = c->h_ptail;
     }
     if (ms) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     if (zh > 0) {
@@ -1983,13 +1970,12 @@ int aon_lds(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
 }
 
 // Per-workgroup scratch rows of the labels-in-HBM kernel (allocated on
-// first use: hbm_grid rows of N labels, rounds, predecessors and tails).
+// first use: hbm_grid rows of N labels, predecessors and tails).
 int ensure_hbm(tapgpu_ctx* c) {
   if (c->h_dist) return TAPGPU_OK;
   const size_t n = (size_t)c->hbm_grid * c->N;
   int rc;
   if ((rc = dalloc(c, &c->h_dist, n))) return rc;
-  if ((rc = dalloc(c, &c->h_rnd, n))) return rc;
   if ((rc = dalloc(c, &c->h_pred, n))) return rc;
   if ((rc = dalloc(c, &c->h_ptail, n))) return rc;
   return TAPGPU_OK;
