@@ -117,3 +117,79 @@ def pred_tie_check(net, costs, dist, pred, o_pred, origins):
         else:
             bad += 1
     return bad, ties
+
+
+def random_network(seed, cost_kind):
+    """A seeded random network with the awkward cases a road file can hold:
+    parallel links, self-loops, zero-cost links (zero-cost cycles with
+    cost_kind "ints"), non-through zones, start nodes that are not zones,
+    costs over twelve orders of magnitude ("wide").  Through nodes are
+    joined by a two-way ring and every non-through node by a two-way link to
+    one through node, so every zone reaches every other."""
+    from traffic_assignment_amd import synth
+    rng = np.random.default_rng(seed)
+    N = int(rng.integers(40, 2500))
+    Z = int(rng.integers(2, min(40, N // 2)))
+    ft = int([1, Z // 2 + 1, Z + 1][seed % 3])
+    E0 = int(N * rng.uniform(1.5, 4.0))
+    init = [rng.integers(1, N + 1, E0)]
+    term = [rng.integers(1, N + 1, E0)]
+    thru = np.arange(ft, N + 1)
+    init += [thru, np.roll(thru, -1)]
+    term += [np.roll(thru, -1), thru]
+    if ft > 1:
+        z = np.arange(1, ft)
+        t = rng.integers(ft, N + 1, z.size)
+        init += [z, t]
+        term += [t, z]
+    dup = rng.integers(0, E0, E0 // 20)                  # parallel links
+    init += [init[0][dup]]
+    term += [term[0][dup]]
+    loops = rng.integers(1, N + 1, 5)                    # self-loops
+    init += [loops]
+    term += [loops]
+    init = np.concatenate(init).astype(np.int64)
+    term = np.concatenate(term).astype(np.int64)
+    order = np.lexsort((term, init))
+    init, term = init[order], term[order]
+    E = init.size
+    if cost_kind == "ints":
+        cost = rng.integers(0, 4, E).astype(np.float64)
+    elif cost_kind == "wide":
+        cost = 10.0 ** rng.uniform(-6, 6, E)
+    else:
+        cost = rng.uniform(0.01, 5.0, E)
+        if cost_kind == "cont":
+            cost[rng.random(E) < 0.15] = 0.0
+    net = synth.Network(num_zones=Z, num_nodes=N, first_thru_node=ft, init=init, term=term,
+                        capacity=rng.uniform(20.0, 200.0, E), length=np.ones(E), fftime=cost,
+                        B=np.full(E, 0.15), power=np.full(E, 4.0), speed=np.zeros(E),
+                        toll=np.zeros(E), ltype=np.ones(E, np.int16))
+    return net, rng
+
+
+def random_demand(net, rng):
+    """Integer demand from every zone to a random subset of the others."""
+    from traffic_assignment_amd import synth
+    Z = net.num_zones
+    dest = [np.sort(rng.choice(np.setdiff1d(np.arange(1, Z + 1), [o]),
+                               size=int(rng.integers(0, Z)), replace=False)) for o in range(1, Z + 1)]
+    start = np.r_[0, np.cumsum([d.size for d in dest])].astype(np.int64)
+    allr = np.concatenate(dest).astype(np.int64)
+    return synth.Demand(Z, start, allr, rng.integers(0, 30, allr.size).astype(np.float64))
+
+
+def pred_reaches_origin(net, dist, pred, origins):
+    """Every reachable node's predecessor chain ends at its origin (the
+    predecessor links form a tree, zero-cost cycles included)."""
+    n1 = net.num_nodes + 1
+    for j, s in enumerate(origins):
+        p = pred[:, j]
+        anc = np.arange(n1)
+        has = p >= 0
+        anc[has] = net.init[p[has]]
+        for _ in range(int(np.ceil(np.log2(n1))) + 1):
+            anc = anc[anc]
+        reach = dist[:, j] < 1.7e308
+        reach[0] = False
+        assert np.all(anc[reach] == s), (s, np.nonzero(anc[reach] != s)[0][:5])

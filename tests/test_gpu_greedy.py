@@ -10,7 +10,7 @@ import tempfile
 import numpy as np
 import pytest
 
-from conftest import b64_array, golden, network
+from conftest import b64_array, golden, network, random_demand, random_network
 
 pytestmark = pytest.mark.gpu
 
@@ -80,7 +80,6 @@ def test_greedy_random_networks(engine_mod, oracle_lib, seed, strategy):
     (parallel links, self-loops, non-through zones): every iteration's
     unsatisfied demand, the volumes and the costs bit-identical to the
     oracle."""
-    from test_gpu_parity import random_demand, random_network
     print(f"case greedy random seed={seed} {strategy}", flush=True)
     net, rng = random_network(seed, "pos")
     dem = random_demand(net, rng)

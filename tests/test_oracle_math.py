@@ -8,7 +8,7 @@ from fractions import Fraction
 import numpy as np
 import pytest
 
-from conftest import network, rmat
+from conftest import network, pred_reaches_origin, random_network, rmat
 
 
 # --- link functions ----------------------------------------------------------
@@ -161,6 +161,48 @@ def test_sssp_fixed_point_invariants(oracle_lib):
     k = p[1:][has]
     assert np.array_equal(d[1:][has], d[net.init[k]] + costs[k])
     assert np.array_equal(net.term[k], np.arange(1, net.num_nodes + 1)[has])
+
+
+@pytest.mark.parametrize("seed,cost_kind", [(1, "cont"), (2, "ints"), (3, "wide"), (4, "ints")])
+def test_sssp_random_networks_fixed_point_and_trees(oracle_lib, seed, cost_kind):
+    """The random networks of the GPU parity tests (zero-cost cycles,
+    parallel links, self-loops, non-through zones): the oracle's labels are
+    the least fixed point, its predecessor links are tight and form a tree
+    per origin, and the tree check catches a planted two-node cycle."""
+    net, rng = random_network(seed, cost_kind)
+    costs = net.fftime
+    origins = np.unique(np.r_[np.arange(1, net.num_zones + 1),
+                              rng.integers(1, net.num_nodes + 1, 10)])
+    n1 = net.num_nodes + 1
+    D = np.empty((n1, origins.size)); P = np.empty((n1, origins.size), np.int64)
+    u, v = net.init, net.term
+    for j, s in enumerate(origins):
+        d, p = oracle_lib.sssp(net, costs, int(s))
+        D[:, j], P[:, j] = d, p
+        ok = ((u >= net.first_thru_node) | (u == s)) & (d[u] < np.finfo(float).max)
+        assert np.all(d[v][ok] <= d[u][ok] + costs[ok])
+        has = p[1:] >= 0
+        k = p[1:][has]
+        assert np.array_equal(d[1:][has], d[u[k]] + costs[k])
+        assert np.all((u[k] >= net.first_thru_node) | (u[k] == s))
+        assert d[s] == 0.0 and p[s] == -1
+    D[0] = np.finfo(float).max; P[0] = -1
+    pred_reaches_origin(net, D, P, origins)
+    # planted mistake: point a node's predecessor at a link from its own child
+    j = 0
+    s = origins[j]
+    plant = None
+    for c in range(1, n1):
+        if P[c, j] >= 0 and u[P[c, j]] != s:
+            back = np.nonzero((u == c) & (v == u[P[c, j]]))[0]
+            if back.size:
+                plant = (u[P[c, j]], back[0])
+                break
+    assert plant is not None
+    bad = P.copy()
+    bad[plant[0], j] = plant[1]
+    with pytest.raises(AssertionError):
+        pred_reaches_origin(net, D, bad, origins)
 
 
 # --- loading -------------------------------------------------------------------
