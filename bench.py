@@ -1,20 +1,23 @@
 """Benchmark: TAP solves/sec to relative gap 1e-4 on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload metro]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batched]
 
-One step = one complete Frank-Wolfe solve (all-or-nothing start at free-flow
-costs, then iterations until relgap <= 1e-4) of the workload, with the
-network and demand already resident in HBM (uploaded before timing).
-Workloads: metro (default: jittered 100x100 grid + 1,000 zones, 43,600
-links), grid5 (24 nodes), regional (300x300 grid + 4,000 zones), batched
-(512 upgrade-subset variants of metro; value = variant solves/sec),
-gradient (equilibrium gradients on metro at relgap 1e-6; value =
-gradients/sec, beside the cost of finite-difference re-solves).
+Default workload (batched): 512 upgrade-subset variants of the metro network
+(jittered 100x100 grid + 1,000 zones, 43,600 links; 32 candidate upgrades,
+project-cost budget 40), all in one engine context.  One step = one
+Frank-Wolfe iteration of the whole batch (every variant's all-or-nothing
+loading in one launch sequence, then the link passes), inputs resident in
+HBM.  W untimed steps, then K timed steps; then, timed on its own, one cold
+solve of the whole batch to relative gap 1e-4 -- value = variant solves/sec
+of that solve.  The same line carries the plain path (a sample of the
+variants solved one at a time on the same GPU), the roofline of the
+dominant kernel (HIP events around every launch of it in the K timed
+steps), the reference's CPU path timed on a bounded sample on this host,
+and the shortest-path GB/s of the labels-in-HBM kernel on the regional
+network.
 
-Prints one JSON line with the throughput, the roofline of the dominant
-kernel (the shortest-path sweep) measured with HIP events around every
-sweep launch in the timed region, and the reference's CPU path timed on a
-bounded sample on this host.
+Other workloads: metro (one plain solve per step), regional, grid5,
+gradient (equilibrium gradients at relgap 1e-6).
 """
 from __future__ import annotations
 
@@ -68,11 +71,37 @@ def origin_bytes(st, num_links) -> float:
     return relax + pred + load
 
 
-def cpu_baseline(net, dem, gpu_iters: int, cores: int):
+def host_cpu():
+    """(cores this process may use, CPU model from lscpu / /proc/cpuinfo)."""
+    cores = len(os.sched_getaffinity(0))
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    if not model:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        model = line.split(":", 1)[1].strip()
+                        break
+        except Exception:
+            pass
+    return cores, model
+
+
+def cpu_baseline(net, dem, gpu_iters: int, cores: int, budget_s: float = 20.0):
     """The reference's tap_frankwolfe_pthread (compiled from its sources into
-    oracle/_ref) on this host: time lo = 2 iterations and hi = lo + k, k
-    chosen for about 20 s of CPU work (2 <= k <= 20); per-iteration time x
-    the GPU's iteration count + setup = one solve (extrapolated)."""
+    oracle/_ref) on this host with `cores` threads: time lo = 2 iterations
+    and hi = lo + k, k chosen for about budget_s of CPU work (2 <= k <= 20);
+    per-iteration time x the GPU's iteration count + setup = one solve
+    (extrapolated).  Without the reference build, the oracle's restatement
+    (bit-identical to it on the golden vectors) is timed instead."""
     from traffic_assignment_amd import tntp
     ref = os.path.join(ROOT, "oracle", "_ref", "tap_frankwolfe_pthread")
     lo = 2
@@ -97,16 +126,27 @@ def cpu_baseline(net, dem, gpu_iters: int, cores: int):
                 oracle.fw_solve(net, dem, max_iters=it, nthreads=cores)
                 return time.perf_counter() - t
         t_lo = run(lo)
-        hi = lo + int(min(20, max(2, 20.0 / max(t_lo / lo, 1e-6))))
+        hi = lo + int(min(20, max(2, budget_s / max(t_lo / lo, 1e-6))))
         t_hi = run(hi)
     per_iter = (t_hi - t_lo) / (hi - lo)
     setup = max(t_lo - lo * per_iter, 0.0)
     solve = setup + per_iter * gpu_iters
     return {"value": 1.0 / solve, "unit": "solves/s", "cores": cores, "kind": kind,
             "sample": (f"{DESCR.get(getattr(net, 'params', {}).get('workload', ''), 'workload')}: "
-                       f"{lo} and {hi} FW iterations timed, {per_iter * 1e3:.1f} ms/iteration, "
-                       f"extrapolated to the GPU's {gpu_iters} iterations + {setup:.2f} s setup"),
+                       f"{lo} and {hi} FW iterations timed on {cores} threads, "
+                       f"{per_iter * 1e3:.1f} ms/iteration, extrapolated to the GPU's "
+                       f"{gpu_iters} iterations + {setup:.2f} s setup"),
             "seconds_per_iteration": per_iter, "measured_seconds": t_lo + t_hi}
+
+
+def cpu_baselines(net, dem, gpu_iters: int):
+    """All host cores (the reported baseline) and one thread beside it."""
+    cores, model = host_cpu()
+    allc = cpu_baseline(net, dem, gpu_iters, cores, 20.0)
+    serial = cpu_baseline(net, dem, gpu_iters, 1, 10.0)
+    allc["cpu_model"] = model
+    allc["serial"] = {k: serial[k] for k in ("value", "seconds_per_iteration", "sample")}
+    return allc
 
 
 def load_traffic(workload):
@@ -120,10 +160,89 @@ def load_traffic(workload):
         return None
 
 
+def lanes_bytes(st, num_nodes, in_degree) -> float:
+    """Algorithmic bytes of the lanes kernel (one workgroup per origin and
+    group of 64 variants), from its counters.  Relaxation, per wavefront
+    expansion (a node in all 64 lanes): its label row 512 B; per out-link
+    scanned: 4 B head, the cost row 512 B and the head's label row 512 B; per
+    lane lowered 8 B.  Label rows set at the start of each task: 512 B per
+    node.  Loading, per lane and walk step: 8 B memo, 4 B tail, 16 B volume
+    read-modify-write; per predecessor selection (lane): 8 B own label and
+    8 B memo, and per in-link 8 B reverse-CSR data, 8 B cost, 8 B tail label
+    (in-links at the network's mean in-degree)."""
+    F, Es, U = st["origin_expansions"], st["origin_scanned"], st["origin_updates"]
+    relax = 512.0 * F + (4.0 + 1024.0) * Es + 8.0 * U
+    init = 512.0 * num_nodes * st["lanes_tasks"]
+    load = 28.0 * st["origin_loaded"] + st["lanes_pred_scans"] * (16.0 + 24.0 * in_degree)
+    return relax + init + load
+
+
+def loading_roofline(acc, path, num_nodes, num_links, traffic=None):
+    """Roofline of the loading kernel from accumulated stats (HIP events
+    around each of its launches)."""
+    launches = acc.get("origin_launches", 0)
+    avg_ms = acc.get("origin_kernel_ms", 0.0) / max(launches, 1)
+    if acc.get("lanes_launches", 0) and acc.get("lanes_launches") == launches:
+        kernel = "k_origin_lanes"
+        total = lanes_bytes(acc, num_nodes, num_links / max(num_nodes, 1))
+    else:
+        kernel = "k_origin_hbm" if path == "hbm" else "k_origin_lds"
+        total = origin_bytes(acc, num_links)
+    bpl = total / max(launches, 1)
+    achieved = bpl / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    return {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2),
+            "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5),
+            "traffic": traffic, "avg_launch_us": round(avg_ms * 1e3, 3),
+            "launches": int(launches), "algorithmic_bytes_per_launch": round(bpl, 1)}
+
+
+def _acc(acc, st):
+    for k, v in st.items():
+        acc[k] = acc.get(k, 0) + v
+
+
+def solve_batch(eng, target, tag, chunk=25):
+    """fw_init, then chunks of FW iterations until every variant has
+    converged (a progress line per chunk); the per-variant results."""
+    eng.fw_init()
+    done = 0
+    while True:
+        done += chunk
+        res = eng.fw_run(max_iters=done, target_relgap=target)
+        running = sum(1 for r in res if not r["converged"] and r["iterations"] >= done)
+        print(f"{tag}: {done} iterations, {running} variants running", file=sys.stderr, flush=True)
+        if running == 0:
+            return res
+
+
+def regional_sssp(args):
+    """Shortest-path GB/s of the labels-in-HBM kernel on the regional network
+    (the SSSP roofline case): a few FW iterations with HIP events around
+    every launch of the loading kernel."""
+    import torch
+    from traffic_assignment_amd import synth
+    from traffic_assignment_amd.engine import TapEngine
+    net, dem = synth.make("regional")
+    with TapEngine(net, dem) as eng:
+        eng.fw_init()                       # bucket-width tuning happens here
+        eng.set_profiling(True)
+        torch.cuda.synchronize()
+        eng.fw_run(max_iters=1 + args.regional_iters)
+        st = eng.stats()
+        path = eng.strategy
+    roof = loading_roofline(st, path, net.num_nodes, net.num_links,
+                            load_traffic("regional"))
+    roof["workload"] = DESCR["regional"]
+    roof["iterations_timed"] = args.regional_iters
+    roof["delta_mean_t0"] = st.get("origin_delta", 0.0)
+    return roof
+
+
 def bench_batched(args):
-    """Batched form: --variants upgrade subsets of the metro network solved to
-    relgap 1e-4 in one batch (one launch per FW step for all variants),
-    against the same engine solving a sample of the variants one at a time."""
+    """Batched form: --variants upgrade subsets of the metro network in one
+    engine context.  K timed steps = K Frank-Wolfe iterations of the whole
+    batch; then one timed cold solve of the batch to relgap 1e-4 (value);
+    then a sample of the variants solved one at a time (plain)."""
     import torch
     from traffic_assignment_amd import variants as V
     from traffic_assignment_amd.engine import TapEngine
@@ -132,72 +251,84 @@ def bench_batched(args):
     eng, u, present = V.make_engine(base, dem, mods, subsets)
     if args.strategy != "auto":
         eng.set_strategy(args.strategy)
+    if args.lanes is not None:
+        eng.set_lanes(args.lanes)
+    # warmup: the initial loading and W iterations (schedule tuning included)
+    eng.fw_init()
+    eng.fw_run(max_iters=args.warmup, target_relgap=TARGET)
+    print(f"warmup: {args.warmup} iterations", file=sys.stderr, flush=True)
     eng.set_profiling(True)
     acc = {}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for step in range(args.steps):
-        eng.fw_init()
-        done = 0
-        while True:         # chunks of 25 iterations, a progress line each
-            done += 25
-            res = eng.fw_run(max_iters=done, target_relgap=TARGET)
-            for k, v in eng.stats().items():
-                acc[k] = acc.get(k, 0) + v
-            running = sum(1 for r in res if not r["converged"] and r["iterations"] >= done)
-            print(f"step {step}: {done} iterations, {running} variants running",
-                  file=sys.stderr, flush=True)
-            if running == 0:
-                break
+        eng.fw_run(max_iters=args.warmup + step + 1, target_relgap=TARGET)
+        _acc(acc, eng.stats())
+        if step % 5 == 4:
+            print(f"step {step + 1}/{args.steps}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    eng.set_profiling(False)
+    # one cold solve of the whole batch to the target
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    res = solve_batch(eng, TARGET, "batch solve")
+    torch.cuda.synchronize()
+    batch_s = time.perf_counter() - t1
     path = eng.strategy
+    lanes_min = eng.lanes
     eng.close()
     iters = [r["iterations"] for r in res]
-    batched = args.steps * len(subsets) / elapsed
-    # plain: the same engine on single variant networks, one after another
+    assert all(r["converged"] for r in res), "a variant did not converge"
+    batched = len(subsets) / batch_s
+    # plain: single-variant engines, one after another (same GPU, default path)
     sample = list(range(0, len(subsets), max(1, len(subsets) // args.plain_sample)))[:args.plain_sample]
-    plain_t = 0.0
+    plain_t, plain_it = 0.0, []
     for v in sample:
         net = V.variant_network(base, mods, subsets[v])
         with TapEngine(net, dem) as pe:
-            if args.strategy != "auto":
-                pe.set_strategy(args.strategy)
             pe.fw_init()            # untimed: schedule tuning and first-launch costs
             pe.fw_run(max_iters=1)
             torch.cuda.synchronize()
             t = time.perf_counter()
             pe.fw_init()
-            pe.fw_run(target_relgap=TARGET)
+            r = pe.fw_run(target_relgap=TARGET)[0]
             torch.cuda.synchronize()
             plain_t += time.perf_counter() - t
+            plain_it.append(r["iterations"])
+            assert r["iterations"] == iters[v], "plain and batched iteration counts differ"
     plain = len(sample) / plain_t
-    launches = acc.get("origin_launches", 0)
-    avg_ms = acc.get("origin_kernel_ms", 0.0) / max(launches, 1)
-    bpl = origin_bytes(acc, u.num_links) / max(launches, 1)
-    achieved = bpl / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    print(f"plain: {len(sample)} variants, {plain:.3f} solves/s", file=sys.stderr, flush=True)
+    roof = loading_roofline(acc, path, u.num_nodes, u.num_links, load_traffic("batched"))
+    sssp = None if args.no_regional else regional_sssp(args)
     cpu = None
     if not args.no_cpu:
-        cores = min(16, os.cpu_count() or 1)
         net0 = V.variant_network(base, mods, subsets[0])
-        c1 = cpu_baseline(net0, dem, int(np.median(iters)), cores)
-        cpu = dict(c1, sample="variant 0 of the batch: " + c1["sample"] + "; per-variant rate")
+        c1 = cpu_baselines(net0, dem, int(iters[0]))
+        cpu = dict(c1, sample="variant 0 of the batch: " + c1["sample"] + "; per-variant rate "
+                   "(the variants are independent solves)")
     out = {
         "metric": METRIC, "value": round(batched, 5), "unit": "solves/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": 0, "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded generator, traffic_assignment_amd/synth.py)",
-        "config": {"workload": f"batched: {len(subsets)} upgrade subsets of metro, "
-                               "32 candidate upgrades, budget 40", "target_relgap": TARGET,
-                   "engine_path": path, "iterations_min_median_max":
-                       [min(iters), int(np.median(iters)), max(iters)]},
-        "plain_solves_per_s": round(plain, 5), "batched_vs_plain": round(batched / plain, 3),
-        "roofline": {"bound": "hbm", "kernel": "k_origin_lds", "achieved": round(achieved, 2),
-                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5),
-                     "traffic": None, "avg_launch_us": round(avg_ms * 1e3, 3),
-                     "launches": int(launches), "algorithmic_bytes_per_launch": round(bpl, 1)},
+        "config": {"workload": f"batched: {len(subsets)} upgrade subsets of metro (11,000 nodes / "
+                               "43,600 links / 1,000 zones), 32 candidate upgrades, budget 40",
+                   "step": f"one Frank-Wolfe iteration of all {len(subsets)} variants",
+                   "target_relgap": TARGET, "engine_path": path, "lanes_min_active": lanes_min,
+                   "iterations_min_median_max": [min(iters), int(np.median(iters)), max(iters)],
+                   "parallelism": "1 GPU"},
+        "batch_solve_s": round(batch_s, 3),
+        "plain_solves_per_s": round(plain, 5), "plain_sample": len(sample),
+        "batched_vs_plain": round(batched / plain, 3),
+        "phase_ms_per_step": {k: round(acc.get(k, 0.0) / args.steps, 3) for k in
+                              ("aon_ms", "link_ms", "total_ms")},
+        "roofline": roof,
         "cpu_baseline": cpu,
+        "sssp_regional": sssp,
     }
+    if sssp:
+        out["sssp_gbps"] = sssp["achieved"]
     if cpu:
         out["gpu_vs_cpu"] = round(batched / cpu["value"], 2)
     print(json.dumps(out))
@@ -280,15 +411,21 @@ def bench_gradient(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="metro",
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="batched",
                     choices=["grid5", "metro", "regional", "batched", "gradient"])
     ap.add_argument("--variants", type=int, default=512, help="batched: number of variants")
     ap.add_argument("--grad-network", default="metro", choices=["grid5", "metro", "regional"],
                     help="gradient: network solved to relgap 1e-6")
-    ap.add_argument("--plain-sample", type=int, default=8,
+    ap.add_argument("--plain-sample", type=int, default=16,
                     help="batched: variants solved one by one for the plain comparison")
+    ap.add_argument("--lanes", type=int, default=None,
+                    help="batched: lanes kernel from this many active variants (0: off)")
+    ap.add_argument("--no-regional", action="store_true",
+                    help="batched: skip the regional shortest-path measurement")
+    ap.add_argument("--regional-iters", type=int, default=2,
+                    help="batched: timed FW iterations of the regional network")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--strategy", default="auto", choices=["auto", "sweeps", "lds", "hbm"])
     ap.add_argument("--chase", type=int, default=0,
@@ -336,6 +473,7 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     eng_path = eng.strategy
+    queue = eng.origin_queue
     eng.close()
 
     value = args.steps / elapsed
@@ -357,8 +495,7 @@ def main():
             "algorithmic_bytes_per_launch": round(bytes_per_launch, 1)}
     cpu = None
     if not args.no_cpu:
-        cores = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(net, dem, int(np.median(iters)), cores)
+        cpu = cpu_baselines(net, dem, int(np.median(iters)))
     out = {
         "metric": METRIC, "value": round(value, 5), "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 3),
@@ -371,7 +508,7 @@ def main():
         "sssp_gbps": round(achieved, 2),
         "schedule": {"origin_chase": acc.get("origin_chase", 0) // max(args.steps, 1),
                      "origin_delta": acc.get("origin_delta", 0.0) / max(args.steps, 1),
-                     "origin_queue": eng.origin_queue},
+                     "origin_queue": queue},
         "origin_phase_us_per_workgroup": {
             k: round(1e3 * acc.get("origin_" + k + "_ms", 0.0) / max(acc.get("origin_blocks", 0), 1), 2)
             for k in ("relax", "pred", "load")},

@@ -138,6 +138,40 @@ void link_volume_data_setup(link_data_t *links, long num_links, long num_start_n
                             demand_data_t *demands[]);
 void link_volume_data_cleanup(void);
 
+/* --- single-source shortest paths (trunk/tap/sssp_pape.h:36-64) ---------------- */
+
+/* The reference's CPU d'Esopo-Pape family, each from one start node on the
+   GPU engine: Va/Ea/Wa the CSR (num_nodes entries of Va + 1; node ids
+   1-based, node 0 without links), prev[v] = link index of the last link of
+   the path to v (-1: none), dist[v] = DBL_MAX where unreachable; nodes below
+   first_thru_node other than the start are labelled but not expanded
+   (trunk/tap/sssp_pape.c:355).  Labels are bit-identical to sssp_pape and
+   sssp_pape_lll; predecessors equal theirs wherever the shortest path is
+   unique and form a tree.  queue_next (the reference's work array) is
+   unused.  The calls are serialised on one cached engine context; for a
+   zone start node the trees of every zone are computed in one launch and
+   kept for the calls with the same costs (the fw_assign loop,
+   trunk/tap/tap_frankwolfe.c:131-147).  Out-of-range start nodes exit(1),
+   as the reference's assert does. */
+void sssp_pape(long Va[], long Ea[], double Wa[], long num_nodes, long num_edges,
+               long start_node, long first_thru_node, long prev[], double dist[],
+               long queue_next[]);                                    /* :36-38 */
+void sssp_slf(long Va[], long Ea[], double Wa[], long num_nodes, long num_edges,
+              long start_node, long first_thru_node, long prev[], double dist[],
+              long queue_next[]);                                     /* :40-43 */
+void sssp_slf_lll(long Va[], long Ea[], double Wa[], long num_nodes, long num_edges,
+                  long start_node, long first_thru_node, long prev[], double dist[],
+                  long queue_next[]);                                 /* :45-48 */
+void sssp_pape_lll(long Va[], long Ea[], double Wa[], long num_nodes, long num_edges,
+                   long start_node, long first_thru_node, long prev[], double dist[],
+                   long queue_next[]);                                /* :50-53 */
+/* trunk/tap/sssp_pape.h:55-56: the path dest .. orig along predecessor nodes */
+void get_shortest_path(long pred[], long orig, long dest, long path[], long *pathlen);
+/* trunk/tap/sssp_pape.h:59-64: also prevnode[v] = tail of prevlink[v] */
+void sssp_prevnodefirst(long Va[], long Ea[], double Wa[], long num_nodes, long num_edges,
+                        long start_node, long first_thru_node, long old_prevnode[],
+                        long prevlink[], long prevnode[], double dist[], long queue_next[]);
+
 /* The reference's greedy assignment program as a library call: the body of
    tap_greedy's main (trunk/tap/tap_greedy.c:296-335) after parsing, with the
    same stderr lines ("initial unsatifised demand", "iter = ...", the

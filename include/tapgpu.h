@@ -201,6 +201,11 @@ typedef struct {
   int64_t origin_split;         /* origins per variant co-run by the HBM     */
                                 /*   kernel beside the LDS kernel            */
   int64_t origin_deferred;      /* queued nodes kept for a later bucket      */
+  /* lanes kernel (batched loading, one variant per wavefront lane) */
+  int64_t lanes_launches;       /* launches                                  */
+  int64_t lanes_tasks;          /* (origin, 64-variant group) workgroups     */
+  int64_t lanes_pred_scans;     /* predecessor selections (lanes)            */
+  int64_t lanes_fallbacks;      /* loadings redone by a per-origin kernel    */
 } tapgpu_stats;
 
 /* Counters and wall times of the last tapgpu_fw_run (or of the calls since
@@ -228,7 +233,7 @@ int tapgpu_set_origin_chase(tapgpu_ctx *ctx, int chase);
 int tapgpu_get_origin_chase(const tapgpu_ctx *ctx);
 
 /* Frontier queues of the one-origin-per-workgroup paths (strategies 2 and
-   3): a round whose frontier (the nodes lowered or kept in the round
+   3) and of the lanes kernel: a round whose frontier (the nodes lowered or kept in the round
    before) has at most this many nodes takes them from an LDS list; a
    larger frontier is found by scanning its bitmap.  -1 (the default): the
    most each kernel's spare LDS holds (up to 4096); 0: bitmap scans only;
@@ -255,6 +260,17 @@ double tapgpu_get_origin_delta(const tapgpu_ctx *ctx);
 int tapgpu_set_origin_split(tapgpu_ctx *ctx, int origins);
 int tapgpu_get_origin_split(const tapgpu_ctx *ctx);
 int tapgpu_get_strategy(const tapgpu_ctx *ctx);
+
+/* Batched loading with the variants of one origin in the lanes of a
+   wavefront (one workgroup per (origin, group of up to 64 variants), labels
+   node-major in HBM, predecessors taken along the demand walks).  A loading
+   of a batch uses it when at least min_active of its variants are active;
+   0: never; -1 (the default): 16 for batched contexts, never for plain ones.
+   A loading that meets a node whose tight in-links all come from tails with
+   its own label (zero-cost links) is redone by the per-origin strategy.
+   Results do not depend on it (same trees, same volumes bit for bit). */
+int tapgpu_set_lanes(tapgpu_ctx *ctx, int min_active);
+int tapgpu_get_lanes(const tapgpu_ctx *ctx);
 
 /* --- greedy step-cost assignment ------------------------------------------ */
 

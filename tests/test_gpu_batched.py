@@ -17,22 +17,32 @@ def _small():
     return base, dem, mods, subsets
 
 
-@pytest.mark.parametrize("strategy", ["sweeps", "lds", "hbm"])
+@pytest.mark.parametrize("strategy", ["sweeps", "lds", "hbm", "lanes"])
 def test_batched_equals_plain_and_oracle(engine_mod, oracle_lib, strategy):
+    """Every loading path; "lanes" puts the 12 variants in the lanes of one
+    group (the per-origin kernel takes over once fewer than 4 are active)."""
     print(f"case batched {strategy}", flush=True)
     base, dem, mods, subsets = _small()
     eng, u, present = variants.make_engine(base, dem, mods, subsets)
     with eng:
-        eng.set_strategy(strategy)
+        if strategy == "lanes":
+            eng.set_lanes(4)
+        else:
+            eng.set_strategy(strategy)
+            eng.set_lanes(0)
         eng.fw_init()
         res = eng.fw_run(target_relgap=1e-4)
         out = [(eng.flows(v), eng.trace(v)) for v in range(len(subsets))]
+        st = eng.stats()
     assert all(r["converged"] for r in res)
+    if strategy == "lanes":
+        assert st["lanes_launches"] > 0 and st["lanes_fallbacks"] == 0
     for v in (0, 3, 7, 11):
         net = variants.variant_network(base, mods, subsets[v])
         m = present[v].astype(bool)
         with engine_mod.TapEngine(net, dem) as pe:
-            pe.set_strategy(strategy)
+            if strategy != "lanes":
+                pe.set_strategy(strategy)
             pe.fw_init()
             pr = pe.fw_run(target_relgap=1e-4)[0]
             px, pc = pe.flows()

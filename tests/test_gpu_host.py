@@ -190,3 +190,101 @@ def test_pape_cuda_fw_assign_link_volume_shims(engine_mod, oracle_lib, hostlib):
                                y2.ctypes.data_as(C.POINTER(C.c_double)), plink.ctypes.data_as(lp))
     hostlib.link_volume_data_cleanup()
     assert np.array_equal(y2, oracle_lib.fw_assign(net, dem, Wa, nthreads=8))
+
+
+def test_load_pred_rejects_non_trees(engine_mod, oracle_lib):
+    """tapgpu_load_pred: the oracle's trees load exactly; a link that does
+    not end at its node, or a two-node predecessor cycle, is an error rather
+    than silently missing demand."""
+    net, dem = network("grid5")
+    od, op = oracle_lib.sssp_all(net, net.fftime)
+    op = np.ascontiguousarray(op, dtype=np.int64)
+    with engine_mod.TapEngine(net, dem) as eng:
+        y = eng.load_pred(op)
+        assert np.array_equal(y, oracle_lib.fw_assign(net, dem, net.fftime, nthreads=2))
+        # a link whose head is another node
+        bad = op.copy()
+        v = 7
+        k = int(np.nonzero(net.term != v)[0][0])
+        bad[v, 0] = k
+        with pytest.raises(engine_mod.TapGpuError, match="ends at node"):
+            eng.load_pred(bad)
+        # a two-node cycle: v's link comes from w and w's from v
+        cyc = op.copy()
+        for k in range(net.num_links):
+            a, b = int(net.init[k]), int(net.term[k])
+            back = np.nonzero((net.init == b) & (net.term == a))[0]
+            if a != 1 and b != 1 and back.size:
+                cyc[b, 0] = k
+                cyc[a, 0] = int(back[0])
+                break
+        with pytest.raises(engine_mod.TapGpuError, match="do not form a tree"):
+            eng.load_pred(cyc)
+        # the context still works after the errors
+        assert np.array_equal(eng.load_pred(op), y)
+
+
+def test_single_source_sssp_entry_points(engine_mod, oracle_lib, hostlib):
+    """sssp_pape / sssp_pape_lll / sssp_slf / sssp_slf_lll /
+    sssp_prevnodefirst (trunk/tap/sssp_pape.h:36-64) through the C ABI:
+    labels bit-identical to the oracle's d'Esopo-Pape, predecessor links
+    equal except at exact ties and forming a tree, for zone origins (the
+    prefetched all-zone trees) and for a through node; new costs are seen;
+    get_shortest_path walks predecessor nodes."""
+    from conftest import pred_reaches_origin, pred_tie_check
+    net, _ = network("metro", 0.3)
+    Va, Ea, Wa = oracle_lib.csr(net)
+    rng = np.random.default_rng(3)
+    N1, E = net.num_nodes + 1, net.num_links
+    lp = C.POINTER(C.c_long)
+    dp = C.POINTER(C.c_double)
+    qn = np.zeros(N1, dtype=np.int64)
+    for trial in range(2):
+        W = np.ascontiguousarray(Wa * rng.uniform(1, 3, Wa.size))
+        starts = [1, 7, net.num_zones, net.first_thru_node + 5, net.num_nodes]
+        for fn in ("sssp_pape_lll", "sssp_pape", "sssp_slf", "sssp_slf_lll"):
+            f = getattr(hostlib, fn)
+            f.restype = None
+            d = np.empty((N1, len(starts))); p = np.empty((N1, len(starts)), dtype=np.int64)
+            for j, s in enumerate(starts):
+                dj = np.full(N1, -1.0); pj = np.full(N1, -5, dtype=np.int64)
+                f(Va.ctypes.data_as(lp), Ea.ctypes.data_as(lp), W.ctypes.data_as(dp), C.c_long(N1),
+                  C.c_long(E), C.c_long(s), C.c_long(net.first_thru_node), pj.ctypes.data_as(lp),
+                  dj.ctypes.data_as(dp), qn.ctypes.data_as(lp))
+                d[:, j], p[:, j] = dj, pj
+            od = np.empty_like(d); op = np.empty_like(p)
+            for j, s in enumerate(starts):
+                od[:, j], op[:, j] = oracle_lib.sssp(net, W, int(s))
+            od[0] = np.finfo(float).max; op[0] = -1
+            assert np.array_equal(d.view(np.int64), od.view(np.int64)), fn
+            bad, ties = pred_tie_check(net, W, d, p, op, starts)
+            assert bad == 0, (fn, bad, ties)
+            pred_reaches_origin(net, d, p, starts)
+    # prevnodefirst: links plus their tail nodes; get_shortest_path on nodes
+    f = hostlib.sssp_prevnodefirst
+    f.restype = None
+    s = 3
+    old = np.full(N1, -1, dtype=np.int64)
+    pl = np.empty(N1, dtype=np.int64); pn = np.empty(N1, dtype=np.int64); dd = np.empty(N1)
+    f(Va.ctypes.data_as(lp), Ea.ctypes.data_as(lp), W.ctypes.data_as(dp), C.c_long(N1), C.c_long(E),
+      C.c_long(s), C.c_long(net.first_thru_node), old.ctypes.data_as(lp), pl.ctypes.data_as(lp),
+      pn.ctypes.data_as(lp), dd.ctypes.data_as(dp), qn.ctypes.data_as(lp))
+    od, _ = oracle_lib.sssp(net, W, s)
+    assert np.array_equal(dd, od)
+    has = pl >= 0
+    assert np.array_equal(pn[has], net.init[pl[has]]) and np.all(pn[~has] == -1)
+    dest = net.num_zones - 2
+    path = np.empty(N1, dtype=np.int64)
+    plen = C.c_long(0)
+    hostlib.get_shortest_path.restype = None
+    hostlib.get_shortest_path(pn.ctypes.data_as(lp), C.c_long(s), C.c_long(dest),
+                              path.ctypes.data_as(lp), C.byref(plen))
+    nodes = path[:plen.value]
+    assert nodes[0] == dest and nodes[-1] == s
+    # the path's links sum to the label (left to right from the origin)
+    acc = 0.0
+    for a, b in zip(nodes[::-1][:-1], nodes[::-1][1:]):
+        k = int(pl[b])
+        assert net.init[k] == a and net.term[k] == b
+        acc = acc + W[k]
+    assert acc == od[dest]

@@ -8,6 +8,8 @@ Outputs (git-ignored, shipped with the tree):
                                            (tap_frankwolfe / fw_assign /
                                            pape_cuda / TNTP parsers)
   traffic_assignment_amd/tap_frankwolfe_hip  CLI (tap_main.c's flags)
+  traffic_assignment_amd/libtapsynth.so    host-threaded free-flow zone times
+                                           for the workload generator
 """
 from __future__ import annotations
 
@@ -56,7 +58,7 @@ def build_host():
         return None
     lib = os.path.join(HERE, "libtap_hip.so")
     _run(["gcc", *C_FLAGS, "-shared", "-I", INC, "-o", lib, *srcs,
-          "-L", HERE, "-ltapgpu", "-Wl,-rpath,$ORIGIN", "-lm"])
+          "-L", HERE, "-ltapgpu", "-Wl,-rpath,$ORIGIN", "-lm", "-lpthread"])
     for src, name in (("tap_main_hip.c", "tap_frankwolfe_hip"),
                       ("tap_greedy_main.c", "tap_greedy_hip")):
         main = os.path.join(CSRC, src)
@@ -67,9 +69,19 @@ def build_host():
     return lib
 
 
+def build_synth():
+    """Host-threaded zone-to-zone free-flow times for the workload generator
+    (no HIP dependency)."""
+    lib = os.path.join(HERE, "libtapsynth.so")
+    _run(["gcc", *C_FLAGS, "-shared", "-o", lib, os.path.join(CSRC, "zone_times.c"), "-lpthread",
+          "-lm"])
+    return lib
+
+
 def build_all():
     build_engine()
     build_host()
+    build_synth()
 
 
 if __name__ == "__main__":

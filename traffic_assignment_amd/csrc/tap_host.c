@@ -6,6 +6,7 @@
  *   fw_assign()          trunk/tap/tap_frankwolfe_cuda.cu:176-275
  *   pape_cuda()          trunk/tap/pape_cuda_host.cu:51-195
  *   link_volume_*()      trunk/tap/volume_update_host.cu:55-247
+ *   sssp_pape*() etc.    trunk/tap/sssp_pape.h:36-64 (one origin per call)
  *
  * The engine context (graph, link records and demand resident in HBM) is
  * created once and cached; the shims that receive raw arrays every call
@@ -15,6 +16,7 @@
 #include <errno.h>
 #include <float.h>
 #include <math.h>
+#include <pthread.h>
 #include <signal.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -254,6 +256,234 @@ void pape_cuda(long Va[], long Ea[], double Wa[], long num_nodes, long num_edges
   free_input(&in);
 }
 
+/* --- single-source shortest paths (sssp_pape.h) ------------------------------- */
+
+/* One engine context for the single-source entry points, built from the CSR
+   the caller passes (Va, Ea; topology only) and kept while the topology is
+   the same; the costs are uploaded when they change.  For a zone origin the
+   trees of every zone are computed in one launch and kept on the host, so
+   that fw_assign's loop over the zones (trunk/tap/tap_frankwolfe.c:131-147)
+   costs one launch per cost vector; other start nodes run on their own.
+   The reference's routines are reentrant; these share the context under a
+   lock. */
+#define SSSP_PREFETCH_MAX (32L * 1024 * 1024)   /* nodes x zones kept on the host */
+
+static pthread_mutex_t g_sssp_lock = PTHREAD_MUTEX_INITIALIZER;
+static struct {
+  tapgpu_ctx *ctx;
+  long num_nodes, num_edges, first_thru;
+  long *Va, *Ea;            /* topology the context was built from */
+  double *Wa;               /* costs last uploaded                  */
+  int costs_set;
+  long zones;               /* zone trees held for these costs (0: none) */
+  double *zd;               /* [num_nodes * zones] node-major        */
+  int64_t *zp;
+} g_sssp;
+
+static void sssp_reset(void)
+{
+  if (g_sssp.ctx) tapgpu_destroy(g_sssp.ctx);
+  free(g_sssp.Va); free(g_sssp.Ea); free(g_sssp.Wa); free(g_sssp.zd); free(g_sssp.zp);
+  memset(&g_sssp, 0, sizeof g_sssp);
+}
+
+/* context for the CSR (Va, Ea) with num_nodes entries (node ids 1..N, row 0
+   unused) */
+static void sssp_context(long Va[], long Ea[], long num_nodes, long num_edges,
+                         long first_thru_node)
+{
+  if (g_sssp.ctx && g_sssp.num_nodes == num_nodes && g_sssp.num_edges == num_edges &&
+      g_sssp.first_thru == first_thru_node &&
+      memcmp(g_sssp.Va, Va, sizeof(long) * (size_t)(num_nodes + 1)) == 0 &&
+      memcmp(g_sssp.Ea, Ea, sizeof(long) * (size_t)(num_edges > 0 ? num_edges : 1)) == 0)
+    return;
+  sssp_reset();
+  if (num_nodes < 2 || num_edges < 0 || Va[0] != 0 || Va[1] != 0 || Va[num_nodes] != num_edges) {
+    fprintf(stderr, "ERROR: shortest paths: bad graph (num_nodes %ld, num_edges %ld; "
+            "node ids are 1-based)\n", num_nodes, num_edges);
+    exit(1);
+  }
+  engine_input in;
+  memset(&in, 0, sizeof in);
+  const long E1 = num_edges + 1;
+  in.init = malloc(sizeof(int64_t) * E1);
+  in.term = malloc(sizeof(int64_t) * E1);
+  in.cap = malloc(sizeof(double) * E1);
+  in.t0 = calloc((size_t)E1, sizeof(double));
+  in.B = calloc((size_t)E1, sizeof(double));
+  in.pw = malloc(sizeof(double) * E1);
+  in.od_start = calloc(1, sizeof(int64_t));
+  in.od_dest = malloc(sizeof(int64_t));
+  in.od_val = malloc(sizeof(double));
+  g_sssp.Va = malloc(sizeof(long) * (size_t)(num_nodes + 1));
+  g_sssp.Ea = malloc(sizeof(long) * (size_t)E1);
+  g_sssp.Wa = malloc(sizeof(double) * (size_t)E1);
+  if (!in.init || !in.term || !in.cap || !in.t0 || !in.B || !in.pw || !in.od_start ||
+      !in.od_dest || !in.od_val || !g_sssp.Va || !g_sssp.Ea || !g_sssp.Wa) {
+    fprintf(stderr, "malloc failed in shortest paths\n");
+    exit(1);
+  }
+  for (long v = 0; v < num_nodes; v++)
+    for (long e = Va[v]; e < Va[v + 1]; e++) {
+      in.init[e] = v;
+      in.term[e] = Ea[e];
+    }
+  for (long e = 0; e < num_edges; e++) {
+    in.cap[e] = 1.0;
+    in.pw[e] = 1.0;
+  }
+  in.net = (tapgpu_network){0, num_nodes - 1, first_thru_node, num_edges, in.init, in.term,
+                            in.cap, in.t0, in.B, in.pw};
+  in.dem = (tapgpu_demand){0, in.od_start, in.od_dest, in.od_val};
+  int rc = tapgpu_create(&g_sssp.ctx, &in.net, &in.dem);
+  free_input(&in);
+  if (rc) die("tapgpu_create", NULL, rc);
+  memcpy(g_sssp.Va, Va, sizeof(long) * (size_t)(num_nodes + 1));
+  memcpy(g_sssp.Ea, Ea, sizeof(long) * (size_t)(num_edges > 0 ? num_edges : 1));
+  g_sssp.num_nodes = num_nodes;
+  g_sssp.num_edges = num_edges;
+  g_sssp.first_thru = first_thru_node;
+}
+
+/* labels and predecessor links from start_node into dist[] / prev[]
+   (num_nodes entries each; node 0 unreachable) */
+static void sssp_run(long Va[], long Ea[], double Wa[], long num_nodes, long num_edges,
+                     long start_node, long first_thru_node, long prev[], double dist[])
+{
+  if (start_node < 1 || start_node >= num_nodes) {
+    fprintf(stderr, "ERROR: shortest paths: start node %ld out of range\n", start_node);
+    exit(1);
+  }
+  pthread_mutex_lock(&g_sssp_lock);
+  sssp_context(Va, Ea, num_nodes, num_edges, first_thru_node);
+  tapgpu_ctx *ctx = g_sssp.ctx;
+  int rc;
+  if (!g_sssp.costs_set || memcmp(g_sssp.Wa, Wa, sizeof(double) * (size_t)num_edges) != 0) {
+    if ((rc = tapgpu_set_costs(ctx, 0, Wa))) die("tapgpu_set_costs", ctx, rc);
+    memcpy(g_sssp.Wa, Wa, sizeof(double) * (size_t)num_edges);
+    g_sssp.costs_set = 1;
+    g_sssp.zones = 0;
+  }
+  const long Z = first_thru_node - 1;
+  if (start_node <= Z && (long)num_nodes * Z <= SSSP_PREFETCH_MAX) {
+    if (g_sssp.zones != Z) {
+      free(g_sssp.zd); free(g_sssp.zp);
+      g_sssp.zd = malloc(sizeof(double) * (size_t)num_nodes * Z);
+      g_sssp.zp = malloc(sizeof(int64_t) * (size_t)num_nodes * Z);
+      int64_t *starts = malloc(sizeof(int64_t) * Z);
+      if (!g_sssp.zd || !g_sssp.zp || !starts) {
+        fprintf(stderr, "malloc failed in shortest paths\n");
+        exit(1);
+      }
+      for (long i = 0; i < Z; i++) starts[i] = i + 1;
+      if ((rc = tapgpu_sssp(ctx, 0, starts, Z, g_sssp.zd, g_sssp.zp, Z, 0)))
+        die("tapgpu_sssp", ctx, rc);
+      free(starts);
+      g_sssp.zones = Z;
+    }
+    for (long v = 0; v < num_nodes; v++) {
+      dist[v] = g_sssp.zd[(size_t)v * Z + start_node - 1];
+      prev[v] = (long)g_sssp.zp[(size_t)v * Z + start_node - 1];
+    }
+  } else {
+    int64_t s = start_node;
+    double *d = malloc(sizeof(double) * (size_t)num_nodes);
+    int64_t *p = malloc(sizeof(int64_t) * (size_t)num_nodes);
+    if (!d || !p) {
+      fprintf(stderr, "malloc failed in shortest paths\n");
+      exit(1);
+    }
+    if ((rc = tapgpu_sssp(ctx, 0, &s, 1, d, p, 1, 0))) die("tapgpu_sssp", ctx, rc);
+    for (long v = 0; v < num_nodes; v++) {
+      dist[v] = d[v];
+      prev[v] = (long)p[v];
+    }
+    free(d); free(p);
+  }
+  pthread_mutex_unlock(&g_sssp_lock);
+}
+
+/* sssp_pape, sssp_pape_lll, sssp_slf, sssp_slf_lll (trunk/tap/sssp_pape.h:36-54):
+   one labelling for all four.  The labels are the least fixed point of
+   fl(dist[u] + Wa[e]) that every one of the reference's deque policies
+   reaches (sssp_slf's float temporaries, trunk/tap/sssp_pape.c:543, are not
+   reproduced: its labels here are the FP64 ones).  prev = link index (-1:
+   none), dist = DBL_MAX where unreachable; queue_next is the reference's
+   work array and is not used. */
+void sssp_pape(long Va[], long Ea[], double Wa[], long num_nodes, long num_edges,
+               long start_node, long first_thru_node, long prev[], double dist[],
+               long queue_next[])
+{
+  (void)queue_next;
+  sssp_run(Va, Ea, Wa, num_nodes, num_edges, start_node, first_thru_node, prev, dist);
+}
+
+void sssp_pape_lll(long Va[], long Ea[], double Wa[], long num_nodes, long num_edges,
+                   long start_node, long first_thru_node, long prev[], double dist[],
+                   long queue_next[])
+{
+  (void)queue_next;
+  sssp_run(Va, Ea, Wa, num_nodes, num_edges, start_node, first_thru_node, prev, dist);
+}
+
+void sssp_slf(long Va[], long Ea[], double Wa[], long num_nodes, long num_edges,
+              long start_node, long first_thru_node, long prev[], double dist[],
+              long queue_next[])
+{
+  (void)queue_next;
+  sssp_run(Va, Ea, Wa, num_nodes, num_edges, start_node, first_thru_node, prev, dist);
+}
+
+void sssp_slf_lll(long Va[], long Ea[], double Wa[], long num_nodes, long num_edges,
+                  long start_node, long first_thru_node, long prev[], double dist[],
+                  long queue_next[])
+{
+  (void)queue_next;
+  sssp_run(Va, Ea, Wa, num_nodes, num_edges, start_node, first_thru_node, prev, dist);
+}
+
+/* sssp_prevnodefirst (trunk/tap/sssp_pape.h:60-65): the same labels and
+   links, plus the tail node of each predecessor link (-1: none).
+   old_prevnode only steers the reference's queue order, not the result. */
+void sssp_prevnodefirst(long Va[], long Ea[], double Wa[], long num_nodes, long num_edges,
+                        long start_node, long first_thru_node, long old_prevnode[],
+                        long prevlink[], long prevnode[], double dist[], long queue_next[])
+{
+  (void)old_prevnode; (void)queue_next;
+  sssp_run(Va, Ea, Wa, num_nodes, num_edges, start_node, first_thru_node, prevlink, dist);
+  for (long v = 0; v < num_nodes; v++) {
+    prevnode[v] = -1;
+    const long e = prevlink[v];
+    if (e < 0) continue;
+    /* the tail of link e: the node whose out-link range holds it */
+    long lo = 0, hi = num_nodes - 1;
+    while (lo < hi) {
+      const long mid = (lo + hi + 1) / 2;
+      if (Va[mid] <= e) lo = mid; else hi = mid - 1;
+    }
+    prevnode[v] = lo;
+  }
+}
+
+/* get_shortest_path (trunk/tap/sssp_pape.h:56-57, sssp_pape.c:237-253): the
+   nodes of the path from dest back to orig along predecessor NODES pred[]
+   (dest first, orig last); pathlen 0 if dest has no predecessor. */
+void get_shortest_path(long pred[], long orig, long dest, long path[], long *pathlen)
+{
+  long v = dest;
+  *pathlen = 0;
+  if (pred[v] == -1) return;
+  while (v != orig) {
+    if (v == -1) {
+      fprintf(stderr, "ERROR: get_shortest_path: no path from %ld to %ld\n", orig, dest);
+      exit(1);
+    }
+    path[(*pathlen)++] = v;
+    v = pred[v];
+  }
+  path[(*pathlen)++] = orig;
+}
+
 /* --- link_volume_* ---------------------------------------------------------------- */
 
 static tapgpu_ctx *g_volume_ctx;
@@ -350,7 +580,7 @@ int tap_frankwolfe(net_data_t *net, demand_data_t **demands, int warm_start_mode
                    int *iterations_at_end, double *relgap_at_end, int time_each_iteration)
 {
   const long E = net->num_links;
-  double *warm = NULL;
+  double *warm = NULL, *warm_cost = NULL;
   if (warm_start_mode) {
     link_flow_t *lf = NULL;
     long nf = 0;
@@ -365,16 +595,25 @@ int tap_frankwolfe(net_data_t *net, demand_data_t **demands, int warm_start_mode
       free(lf);
       return 1;
     }
-    /* links added since the flows were written start at zero volume */
+    /* links added since the flows were written start at zero volume and
+       cost (trunk/tap/tap_frankwolfe_pthread.c:564-590) */
     warm = calloc((size_t)E + 1, sizeof(double));
+    warm_cost = calloc((size_t)E + 1, sizeof(double));
+    if (!warm || !warm_cost) {
+      fprintf(stderr, "malloc failed in tap_frankwolfe\n");
+      free(lf); free(warm); free(warm_cost);
+      return 1;
+    }
     long j = 0;
     for (long k = 0; k < E; k++)
       if (j < nf && lf[j].init_node == net->links[k].init_node &&
-          lf[j].term_node == net->links[k].term_node)
+          lf[j].term_node == net->links[k].term_node) {
+        warm_cost[k] = lf[j].cost;
         warm[k] = lf[j++].volume;
+      }
     if (j != nf) {
       fprintf(stderr, "error: flows input file does not match the network links\n");
-      free(lf); free(warm);
+      free(lf); free(warm); free(warm_cost);
       return 1;
     }
     free(lf);
@@ -383,14 +622,14 @@ int tap_frankwolfe(net_data_t *net, demand_data_t **demands, int warm_start_mode
   engine_input in;
   if (make_input(net, demands, &in)) {
     fprintf(stderr, "malloc failed in tap_frankwolfe\n");
-    free(warm);
+    free(warm); free(warm_cost);
     return 1;
   }
   tapgpu_ctx *ctx = NULL;
   int rc = tapgpu_create(&ctx, &in.net, &in.dem);
   if (rc) {
     fprintf(stderr, "ERROR: %s\n", tapgpu_last_error(NULL));
-    free_input(&in); free(warm);
+    free_input(&in); free(warm); free(warm_cost);
     return 1;
   }
   double *x = malloc(sizeof(double) * (E + 1)), *c = malloc(sizeof(double) * (E + 1));
@@ -412,8 +651,18 @@ int tap_frankwolfe(net_data_t *net, demand_data_t **demands, int warm_start_mode
     goto done;
   }
   if (warm) {
-    /* the file's costs should be BPR(volume) (reference warns otherwise) */
-    tapgpu_get_flows(ctx, 0, x, c);
+    /* the file's costs should be BPR(volume): warn where they differ by
+       more than EPS (trunk/tap/tap_frankwolfe_pthread.c:682-690) */
+    if ((rc = tapgpu_get_flows(ctx, 0, x, c))) {
+      fprintf(stderr, "ERROR: %s\n", tapgpu_last_error(ctx));
+      ret = 1;
+      goto done;
+    }
+    for (long k = 0; k < E; k++)
+      if (fabs(c[k] - warm_cost[k]) > 1e-08)
+        fprintf(stderr, "warning: mismatched costs (delta = %g) on link %d from %ld to %ld\n",
+                fabs(c[k] - warm_cost[k]), (int)k, net->links[k].init_node,
+                net->links[k].term_node);
   }
   /* the device applies the loop test of tap_frankwolfe_pthread.c:712-714
      every iteration; the host looks at signals between chunks of iterations
@@ -486,7 +735,7 @@ done:
     sigaction(sigs[i], &old[i], NULL);
   tapgpu_destroy(ctx);
   free_input(&in);
-  free(warm); free(x); free(c);
+  free(warm); free(warm_cost); free(x); free(c);
   if (demands) {
     for (long o = 0; o < net->num_zones + 1; o++)
       free(demands[o]);

@@ -378,6 +378,26 @@ __global__ __launch_bounds__(kBlock) void k_aon_level(Graph G, Labels L, Loading
   block_flush(F, &s_any, &F.flags[r + 1], c_items, c_lanes, 0, 0, s_cnt);
 }
 
+// After the level rounds have drained every lane of a tree is done (-1).  A
+// lane still waiting for children (>= 0) or released but never run (<= -2)
+// lies on a predecessor cycle or hangs from one: flag it (err code 4).
+__global__ __launch_bounds__(kBlock) void k_aon_check(Graph G, Labels L, Loading A, int g0, int Gn) {
+  const long long n = (long long)G.N * Gn * kWave;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock) {
+    const long long item = i / kWave;
+    const int v = (int)(item / Gn);
+    const int g = g0 + (int)(item - (long long)v * Gn);
+    if (!group_on(L, g)) continue;
+    const int col = g * kWave + (int)(i % kWave);
+    if (L.col_src[col] < 0) continue;
+    if (L.pend[(size_t)v * L.P + col] != -1 && atomicCAS(&A.err[0], 0, 4) == 0) {
+      A.err[1] = L.col_src[col];
+      A.err[2] = v;
+    }
+  }
+}
+
 // y[var][e] = sum over the variant's groups of ypart (fixed order); clears
 // ypart for the next loading.
 __global__ __launch_bounds__(kBlock) void k_aon_reduce(int E, int Gv, int v0, int Vn,
@@ -1477,6 +1497,7 @@ struct tapgpu_ctx {
   std::vector<int> h_od_dest;
   std::vector<double> h_od_val;
   std::vector<int> h_tail;            // tail node of each link (0-based)
+  std::vector<int> h_head;            // head node of each link (0-based)
   bool lds_ok = false;
   size_t lds_bytes = 0;
   int qcap = 0;                       // LDS frontier queue entries in use (per buffer)
@@ -1522,6 +1543,19 @@ struct tapgpu_ctx {
   // out like pin_block, read back with one copy
   unsigned char* iter_dev = nullptr;
   size_t iter_bytes = 0;
+  // lanes kernel (batched loading, variants in wavefront lanes)
+  int lanes_min = 0;                  // active variants needed to use it (0: never)
+  bool lanes_ok = false;              // its LDS (bitmaps + queues) fits
+  int lanes_qcap = 0;                 // LDS frontier queue entries per round
+  int lanes_qcap_max = 0;             // entries its LDS allocation holds
+  size_t lanes_lds = 0;
+  int lanes_grid = 0;                 // persistent workgroups (rows allocated)
+  int lanes_grid_max = 0;             // resident workgroups on this device
+  unsigned long long *l_dist = nullptr, *l_memo = nullptr;   // [grid][N][64]
+  double *l_W = nullptr, *l_Y = nullptr;                     // [groups][E][64]
+  int* l_lane_var = nullptr;          // [groups * 64]
+  std::vector<int> h_lane_var;        // its host copy (as uploaded)
+  unsigned l_stamp = 1;               // stamp of the next task's memo entries
   // stats
   tapgpu_stats stats{};
   bool profiling = false;
@@ -1711,8 +1745,11 @@ int sssp_groups(tapgpu_ctx* c, int g0, int Gn, const int* col_src, const int* gr
   return rc;
 }
 
-// Loading for groups [g0, g0+Gn) of variants [v0, v0+Vn) into y.
-int aon_groups(tapgpu_ctx* c, int g0, int Gn, int v0, int Vn, const unsigned char* var_on) {
+// Loading for groups [g0, g0+Gn) of variants [v0, v0+Vn) into y.  With
+// check_tree the predecessor links came from the caller: lanes that never
+// drained (a cycle) are reported as TAPGPU_E_ARG.
+int aon_groups(tapgpu_ctx* c, int g0, int Gn, int v0, int Vn, const unsigned char* var_on,
+               bool check_tree = false) {
   int rc = ensure_labels(c);
   if (rc) return rc;
   rc = maybe_reset_tags(c);
@@ -1737,12 +1774,20 @@ int aon_groups(tapgpu_ctx* c, int g0, int Gn, int v0, int Vn, const unsigned cha
       &c->stats.aon_levels, &c->stats.level_kernel_ms, &c->stats.level_launches);
   if (rc) return rc;
   if ((rc = read_counters(c, &c->stats.aon_items, &c->stats.aon_lanes, nullptr, nullptr))) return rc;
+  if (check_tree) {
+    hipLaunchKernelGGL(k_aon_check, dim3(grid_for((long long)c->N * Gn)), dim3(kBlock), 0,
+                       c->stream, G, L, A, g0, Gn);
+    HIPCHK(c, hipGetLastError());
+  }
   int herr[3];
   HIPCHK(c, hipMemcpyAsync(herr, c->err_dev, sizeof herr, hipMemcpyDeviceToHost, c->stream));
   hipLaunchKernelGGL(k_aon_reduce, dim3(grid_for((long long)c->E * Vn / kWave + 1)), dim3(kBlock), 0,
                      c->stream, c->E, c->Gv, v0, Vn, var_on, c->ypart, c->y);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->stats.aon_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (herr[0] == 4)
+    return c->fail(TAPGPU_E_ARG, "predecessor links do not form a tree (origin %d, node %d)",
+                   herr[1] + 1, herr[2] + 1);
   if (herr[0])
     return c->fail(TAPGPU_E_NOPATH, "no path from %d to %d", herr[1] + 1, herr[2] + 1);
   return TAPGPU_OK;
@@ -2048,9 +2093,39 @@ int aon_hbm(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
   return finish_origin(c, Vn, t0);
 }
 
+#include "taplanes.inc"
+
+// The lanes kernel loads a batch when at least lanes_min of its variants
+// are active (lanes_min 0: never).
+bool use_lanes(tapgpu_ctx* c, int v0, int Vn, const unsigned char* h_on) {
+  if (c->lanes_min <= 0 || !c->lanes_ok) return false;
+  int n = 0;
+  for (int v = v0; v < v0 + Vn; ++v) n += (!h_on || h_on[v]) ? 1 : 0;
+  return n >= c->lanes_min;
+}
+
 // All-or-nothing loading of variants [v0, v0 + Vn) (groups of those
-// variants) with the current costs, by the configured strategy.
-int load_variants(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
+// variants) with the current costs, by the configured strategy.  h_on is
+// the host copy of var_on (null: every variant of the range).
+int load_variants(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on,
+                  const unsigned char* h_on) {
+  if (use_lanes(c, v0, Vn, h_on)) {
+    bool fallback = false;
+    int rc = aon_lanes(c, v0, Vn, var_on, h_on, &fallback);
+    if (rc || !fallback) return rc;
+    // a level tie: the per-origin kernels redo the loading from cleared
+    // volumes, error flag and counters
+    const bool pc = c->pre_cleared;
+    c->pre_cleared = false;
+    if (use_lds(c)) rc = aon_lds(c, v0, Vn, var_on);
+    else if (use_hbm(c)) rc = aon_hbm(c, v0, Vn, var_on);
+    else {
+      rc = sssp_groups(c, v0 * c->Gv, Vn * c->Gv, c->col_src, c->grp_var, var_on);
+      if (!rc) rc = aon_groups(c, v0 * c->Gv, Vn * c->Gv, v0, Vn, var_on);
+    }
+    c->pre_cleared = pc;
+    return rc;
+  }
   if (use_lds(c)) return aon_lds(c, v0, Vn, var_on);
   if (use_hbm(c)) return aon_hbm(c, v0, Vn, var_on);
   int rc = sssp_groups(c, v0 * c->Gv, Vn * c->Gv, c->col_src, c->grp_var, var_on);
@@ -2265,6 +2340,33 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
         hipSuccess || per_cu < 1)
       per_cu = 1;
     c->hbm_grid = std::max(1, ncu * per_cu);
+    // lanes kernel: three bitmaps, then queues from the LDS share of two
+    // workgroups per CU
+    {
+      int lds_cu = 0;
+      if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
+              hipSuccess || lds_cu <= 0)
+        lds_cu = lds_max;
+      const size_t base = lanes_lds_bytes(c->N, 0);
+      c->lanes_ok = base + 256 <= (size_t)lds_max;
+      if (c->lanes_ok) {
+        const size_t share = std::min<size_t>((size_t)lds_cu / 2, (size_t)lds_max) - 256;
+        const size_t spare = share > base ? share - base : 0;
+        c->lanes_qcap_max = (int)std::min<size_t>(kLanesQueueMax, spare / 8) & ~63;
+        c->lanes_qcap = c->lanes_qcap_max;
+        c->lanes_lds = lanes_lds_bytes(c->N, c->lanes_qcap_max);
+        int pc = 0;
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(k_origin_lanes),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lanes_lds) !=
+                hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k_origin_lanes, kLanesBlock,
+                                                          c->lanes_lds) != hipSuccess ||
+            pc < 1)
+          pc = 1;
+        c->lanes_grid_max = std::max(1, ncu * pc);
+      }
+      c->lanes_min = V > 1 ? kLanesMinDefault : 0;
+    }
     double sum = 0;
     int cnt = 0;
     for (int64_t k = 0; k < E; ++k) {
@@ -2367,6 +2469,7 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
   c->h_od_dest = std::move(h_od_dest);
   c->h_od_val = std::move(h_od_val);
   c->h_tail = std::move(tail);
+  c->h_head = fw_dst;
 
   HIPCHK(c, hipMemsetAsync(c->x, 0, sizeof(double) * VE, c->stream));
   HIPCHK(c, hipMemsetAsync(c->y, 0, sizeof(double) * VE, c->stream));
@@ -2484,6 +2587,13 @@ int tapgpu_sssp(tapgpu_ctx* c, int64_t variant, const int64_t* start_nodes, int6
       const int64_t ns = std::min<int64_t>(chunk, num_start - s0);
       for (int64_t j = 0; j < ns; ++j) hs[j] = (int)(start_nodes[s0 + j] - 1);
       if ((rc = upload(c, d_src, hs.data(), ns))) break;
+      // a failed launch must not hand back stale links: predecessors start
+      // at -1 and the error flag at 0 for every chunk
+      if (hipMemsetAsync(d_pred, 0xFF, sizeof(int) * (size_t)c->N * chunk, c->stream) != hipSuccess ||
+          hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream) != hipSuccess) {
+        rc = c->fail(TAPGPU_E_HIP, "clearing the shortest-path outputs failed");
+        break;
+      }
       OriginArgs A = make_origin_args(c, nullptr);
       A.fww += (size_t)variant * c->E; A.rvw += (size_t)variant * c->E;
       A.src_list = d_src; A.out_dist = d_dist; A.out_pred = d_pred; A.ld = chunk;
@@ -2493,15 +2603,34 @@ int tapgpu_sssp(tapgpu_ctx* c, int64_t variant, const int64_t* start_nodes, int6
       else
         hipLaunchKernelGGL(k_origin_lds, dim3((unsigned)ns, 1), dim3(kLdsBlock), c->lds_bytes,
                            c->stream, A, 0);
+      int herr[3] = {0, 0, 0};
       const hipError_t e0 = hipGetLastError();
       const hipError_t e1 = hipMemcpyAsync(hd.data(), d_dist, sizeof(double) * c->N * chunk,
                                            hipMemcpyDeviceToHost, c->stream);
       const hipError_t e2 = hipMemcpyAsync(hp.data(), d_pred, sizeof(int) * c->N * chunk,
                                            hipMemcpyDeviceToHost, c->stream);
+      const hipError_t e4 = hipMemcpyAsync(herr, c->err_dev, sizeof herr, hipMemcpyDeviceToHost,
+                                           c->stream);
       const hipError_t e3 = hipStreamSynchronize(c->stream);
-      if (e0 != hipSuccess || e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
-        rc = c->fail(TAPGPU_E_HIP, "shortest-path launch failed: %s", hipGetErrorString(
-                         e0 != hipSuccess ? e0 : (e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3))));
+      if (e0 != hipSuccess || e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess ||
+          e4 != hipSuccess) {
+        const hipError_t e = e0 != hipSuccess ? e0 : e1 != hipSuccess ? e1 : e2 != hipSuccess ? e2
+                             : e4 != hipSuccess ? e4 : e3;
+        rc = c->fail(TAPGPU_E_HIP, "shortest-path launch failed: %s", hipGetErrorString(e));
+        break;
+      }
+      if (herr[0] == 2) {
+        rc = c->fail(TAPGPU_E_LIMIT, "label-correcting rounds exceeded from origin %d (negative costs?)",
+                     herr[1] + 1);
+        break;
+      }
+      if (herr[0] == 3) {
+        rc = c->fail(TAPGPU_E_LIMIT, "predecessor cycle in the tree of origin %d", herr[1] + 1);
+        break;
+      }
+      if (herr[0] != 0) {
+        rc = c->fail(TAPGPU_E_LIMIT, "shortest-path kernel reported error %d at origin %d", herr[0],
+                     herr[1] + 1);
         break;
       }
       for (int64_t v = 1; v < N1; ++v)
@@ -2571,7 +2700,7 @@ int tapgpu_aon(tapgpu_ctx* c, int64_t variant, double* volumes) {
   unsigned char* d_on = nullptr;
   if ((rc = dalloc(c, &d_on, c->V))) return rc;
   rc = upload(c, d_on, on.data(), c->V);
-  if (!rc) rc = load_variants(c, v, 1, d_on);
+  if (!rc) rc = load_variants(c, v, 1, d_on, on.data());
   if (!rc && volumes) {
     const hipError_t e = hipMemcpy(volumes, c->y + (size_t)v * c->E, sizeof(double) * c->E,
                                    hipMemcpyDeviceToHost);
@@ -2620,7 +2749,7 @@ int tapgpu_fw_init(tapgpu_ctx* c, const double* warm_volumes) {
     if ((rc = upload(c, c->x, warm_volumes, (size_t)c->V * c->E))) return rc;
   } else {
     if ((rc = launch_costs(c, 0, nullptr, 0, c->V))) return rc;   // Wa = t0
-    if ((rc = load_variants(c, 0, c->V, nullptr))) return rc;
+    if ((rc = load_variants(c, 0, c->V, nullptr, nullptr))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->x, c->y, sizeof(double) * (size_t)c->V * c->E,
                              hipMemcpyDeviceToDevice, c->stream));
   }
@@ -2683,7 +2812,7 @@ int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_res
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
     c->defer_origin = true;
     c->pre_cleared = tuned;
-    rc = load_variants(c, 0, c->V, c->var_on);
+    rc = load_variants(c, 0, c->V, c->var_on, hon.data());
     c->defer_origin = false;
     c->pre_cleared = false;
     if (rc) return rc;
@@ -2798,6 +2927,9 @@ int tapgpu_load_pred(tapgpu_ctx* c, int64_t variant, const int64_t* predlink, in
       const int64_t e = predlink[(size_t)node * ld + col_base + c->zone_order[j]];
       if (e < -1 || e >= c->E)
         return c->fail(TAPGPU_E_ARG, "predecessor link %lld out of range", (long long)e);
+      if (e >= 0 && c->h_head[e] != node - 1)
+        return c->fail(TAPGPU_E_ARG, "predecessor link %lld of node %lld (origin %d) ends at node %d",
+                       (long long)e, (long long)node, c->zone_order[j] + 1, c->h_head[e] + 1);
       hp[(size_t)(node - 1) * c->Pv + j] = (int)e;
     }
   HIPCHK(c, hipMemcpy2DAsync(c->pred + (size_t)v * c->Pv, c->P * sizeof(int), hp.data(),
@@ -2808,7 +2940,7 @@ int tapgpu_load_pred(tapgpu_ctx* c, int64_t variant, const int64_t* predlink, in
   unsigned char* d_on = nullptr;
   if ((rc = dalloc(c, &d_on, c->V))) return rc;
   rc = upload(c, d_on, on.data(), c->V);
-  if (!rc) rc = aon_groups(c, v * c->Gv, c->Gv, v, 1, d_on);
+  if (!rc) rc = aon_groups(c, v * c->Gv, c->Gv, v, 1, d_on, true);
   if (!rc && volumes) {
     const hipError_t e = hipMemcpy(volumes, c->y + (size_t)v * c->E, sizeof(double) * c->E,
                                    hipMemcpyDeviceToHost);
@@ -2883,6 +3015,7 @@ int tapgpu_set_origin_queue(tapgpu_ctx* c, int entries) {
   if (!c || entries < -1) return TAPGPU_E_ARG;
   c->qcap = entries < 0 ? c->qcap_max : std::min(entries, c->qcap_max);
   c->hqcap = entries < 0 ? c->hqcap_max : std::min(entries, c->hqcap_max);
+  c->lanes_qcap = entries < 0 ? c->lanes_qcap_max : std::min(entries, c->lanes_qcap_max);
   return TAPGPU_OK;
 }
 
@@ -2897,6 +3030,16 @@ int tapgpu_get_strategy(const tapgpu_ctx* c) {
   tapgpu_ctx* m = const_cast<tapgpu_ctx*>(c);
   return use_lds(m) ? 2 : (use_hbm(m) ? 3 : 1);
 }
+
+int tapgpu_set_lanes(tapgpu_ctx* c, int min_active) {
+  if (!c || min_active < -1) return TAPGPU_E_ARG;
+  if (min_active != 0 && !c->lanes_ok)
+    return c->fail(TAPGPU_E_ARG, "frontier bitmaps of this network do not fit LDS");
+  c->lanes_min = min_active < 0 ? (c->V > 1 ? kLanesMinDefault : 0) : min_active;
+  return TAPGPU_OK;
+}
+
+int tapgpu_get_lanes(const tapgpu_ctx* c) { return c ? c->lanes_min : TAPGPU_E_ARG; }
 
 int tapgpu_set_profiling(tapgpu_ctx* c, int on) {
   if (!c) return TAPGPU_E_ARG;

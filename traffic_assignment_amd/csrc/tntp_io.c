@@ -355,6 +355,8 @@ int apply_trip_modification(demand_data_t *demands[], trip_mod_t *mod, long num_
   }
   switch (mod->trip_modtype) {
   case TRIP_MOD_TYPE_POINT_E:
+    fprintf(stderr, "[tripchange %s] %ld -> %ld multiplier %f\n", mod->trip_change_id,
+            mod->origin, mod->dest, mod->multiplier);
     for (long i = 0; demands[mod->origin][i].dest != 0; i++)
       if (demands[mod->origin][i].dest == mod->dest) {
         demands[mod->origin][i].demand *= mod->multiplier;
@@ -363,11 +365,15 @@ int apply_trip_modification(demand_data_t *demands[], trip_mod_t *mod, long num_
     return 0;
   case TRIP_MOD_TYPE_UNIFORM_E:
     if (mod->dest == 0 && mod->origin > 0) {
+      fprintf(stderr, "[tripchange %s] origin %ld multiplier %f\n", mod->trip_change_id,
+              mod->origin, mod->multiplier);
       for (long i = 0; demands[mod->origin][i].dest != 0; i++)
         demands[mod->origin][i].demand *= mod->multiplier;
       return 0;
     }
     if (mod->origin == 0 && mod->dest > 0) {
+      fprintf(stderr, "[tripchange %s] dest %ld multiplier %f\n", mod->trip_change_id,
+              mod->dest, mod->multiplier);
       for (long o = 1; o <= num_zones; o++)
         for (long i = 0; demands[o][i].dest != 0; i++)
           if (demands[o][i].dest == mod->dest)
@@ -377,6 +383,7 @@ int apply_trip_modification(demand_data_t *demands[], trip_mod_t *mod, long num_
     fprintf(stderr, "ERROR: both origin and dest nonzero in uniform trip mod\n");
     return -1;
   case TRIP_MOD_TYPE_ALL_E:
+    fprintf(stderr, "[tripchange %s] ALL multiplier %f\n", mod->trip_change_id, mod->multiplier);
     for (long o = 1; o <= num_zones; o++)
       for (long i = 0; demands[o][i].dest != 0; i++)
         demands[o][i].demand *= mod->multiplier;

@@ -70,7 +70,9 @@ class _Stats(C.Structure):
                 ("origin_relax_ms", C.c_double), ("origin_pred_ms", C.c_double),
                 ("origin_load_ms", C.c_double), ("origin_tune_ms", C.c_double),
                 ("origin_chase", C.c_int64), ("origin_delta", C.c_double),
-                ("origin_split", C.c_int64), ("origin_deferred", C.c_int64)]
+                ("origin_split", C.c_int64), ("origin_deferred", C.c_int64),
+                ("lanes_launches", C.c_int64), ("lanes_tasks", C.c_int64),
+                ("lanes_pred_scans", C.c_int64), ("lanes_fallbacks", C.c_int64)]
 
 
 class _GreedyResult(C.Structure):
@@ -106,6 +108,7 @@ def lib():
         L.tapgpu_set_costs.argtypes = [vp, C.c_int64, vp]
         L.tapgpu_sssp.argtypes = [vp, C.c_int64, vp, C.c_int64, vp, vp, C.c_int64, C.c_int64]
         L.tapgpu_aon.argtypes = [vp, C.c_int64, vp]
+        L.tapgpu_load_pred.argtypes = [vp, C.c_int64, vp, C.c_int64, C.c_int64, vp]
         L.tapgpu_fw_init.argtypes = [vp, vp]
         L.tapgpu_fw_run.argtypes = [vp, C.c_int, C.c_double, vp]
         L.tapgpu_get_flows.argtypes = [vp, C.c_int64, vp, vp]
@@ -121,6 +124,8 @@ def lib():
         L.tapgpu_get_origin_queue.argtypes = [vp]
         L.tapgpu_set_origin_delta.argtypes = [vp, C.c_double]
         L.tapgpu_set_origin_split.argtypes = [vp, C.c_int]
+        L.tapgpu_set_lanes.argtypes = [vp, C.c_int]
+        L.tapgpu_get_lanes.argtypes = [vp]
         L.tapgpu_greedy.argtypes = [vp, C.c_int64, C.c_int, vp, vp, vp, vp, C.c_int,
                                     C.POINTER(_GreedyResult)]
         L.tapgpu_get_origin_split.argtypes = [vp]
@@ -222,6 +227,16 @@ class TapEngine:
     def aon(self, variant=0):
         y = np.empty(self.E, dtype=np.float64)
         self._check(lib().tapgpu_aon(self._h, variant, _p(y)))
+        return y
+
+    def load_pred(self, pred, variant=0):
+        """All-or-nothing loading along caller-supplied trees
+        (tapgpu_load_pred): pred is [(N+1), Z] (row = node id, column =
+        origin zone - 1) of the last link on each path, -1 = none."""
+        p = np.ascontiguousarray(pred, dtype=np.int64)
+        assert p.shape == (self.N + 1, self.Z)
+        y = np.empty(self.E, dtype=np.float64)
+        self._check(lib().tapgpu_load_pred(self._h, variant, _p(p), self.Z, 0, _p(y)))
         return y
 
     # -- Frank-Wolfe --
@@ -332,6 +347,15 @@ class TapEngine:
     @property
     def origin_delta(self):
         return lib().tapgpu_get_origin_delta(self._h)
+
+    def set_lanes(self, min_active):
+        """Batched loading with variants in wavefront lanes when at least
+        min_active variants are active (0: never, -1: the default)."""
+        self._check(lib().tapgpu_set_lanes(self._h, int(min_active)))
+
+    @property
+    def lanes(self):
+        return lib().tapgpu_get_lanes(self._h)
 
     def set_profiling(self, on=True):
         self._check(lib().tapgpu_set_profiling(self._h, int(bool(on))))

@@ -19,9 +19,11 @@ Workloads (all float64, all seeded, every free parameter recorded in
                  shortest-path time as impedance.
 * ``regional``   jittered 300x300 grid + 2,000 bidirectional arterial
                  shortcuts (5-20 km) + 4,000 zones with 3 connectors each;
-                 gravity/Poisson demand with a straight-line impedance proxy
-                 (an exact 4,000-source free-flow APSP is too slow to run
-                 at generation time).
+                 gravity/Poisson trip counts with the free-flow zone-to-zone
+                 shortest-path time as impedance, scaled so that 30% of the
+                 16M OD pairs are nonzero; the assigned demand is counts x
+                 2^-6 (peak-hour share; a power of two, so every partial sum
+                 of demand stays exact in FP64).
 * ``upgrades``   32 candidate upgrades on a metro network and a list of
                  budgeted subsets (the batched-variant workload).
 
@@ -196,6 +198,32 @@ def _connectors(zone_ids, grid_node_ids, xy):
                 ltype=np.zeros(2 * n, dtype=np.int16))
 
 
+def _zone_times_native(net: Network) -> np.ndarray:
+    """Zone-to-zone free-flow times on host threads (libtapsynth.so, built
+    by traffic_assignment_amd.build): the same rule as freeflow_zone_times
+    (zones other than the source are not expanded), for networks where a
+    single-threaded library Dijkstra over every zone is too slow."""
+    import ctypes as C
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtapsynth.so")
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built (run traffic_assignment_amd.build)")
+    L = C.CDLL(path)
+    L.tap_zone_times.argtypes = [C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_int64, C.c_int64, C.c_void_p, C.c_int]
+    Z = net.num_zones
+    out = np.empty((Z, Z), dtype=np.float64)
+    a = np.ascontiguousarray(net.init, dtype=np.int64)
+    b = np.ascontiguousarray(net.term, dtype=np.int64)
+    w = np.ascontiguousarray(net.fftime, dtype=np.float64)
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    rc = L.tap_zone_times(net.num_nodes, net.num_links, a.ctypes.data, b.ctypes.data,
+                          w.ctypes.data, Z, net.first_thru_node, out.ctypes.data, threads)
+    if rc != 0:
+        raise RuntimeError("tap_zone_times failed")
+    return out
+
+
 def freeflow_zone_times(net: Network) -> np.ndarray:
     """Zone-to-zone free-flow shortest-path times, zones not traversable.
 
@@ -214,13 +242,28 @@ def freeflow_zone_times(net: Network) -> np.ndarray:
     return d[:, N:N + Z]
 
 
-def _gravity(rng, Z, t0, total, beta=0.1):
+def _gravity(rng, Z, t0, total, beta=0.1, nonzero=None):
+    """Poisson OD counts with mean k * P_i * A_j * exp(-beta * t0_ij), P and
+    A ~ LogNormal(5, 1).  k makes the expected total `total`, or (nonzero
+    given) the expected share of nonzero off-diagonal pairs `nonzero`
+    (bisection on k, in log space)."""
     P = rng.lognormal(5.0, 1.0, size=Z)
     A = rng.lognormal(5.0, 1.0, size=Z)
     w = P[:, None] * A[None, :] * np.exp(-beta * t0)
     np.fill_diagonal(w, 0.0)
     w[~np.isfinite(w)] = 0.0
-    k = total / w.sum()
+    if nonzero is None:
+        k = total / w.sum()
+    else:
+        ww = w[~np.eye(Z, dtype=bool)]
+        lo, hi = -40.0, 10.0                  # log k
+        for _ in range(40):
+            mid = 0.5 * (lo + hi)
+            if np.mean(-np.expm1(-np.exp(mid) * ww)) < nonzero:
+                lo = mid
+            else:
+                hi = mid
+        k = float(np.exp(0.5 * (lo + hi)))
     od = rng.poisson(k * w).astype(np.float64)
     np.fill_diagonal(od, 0.0)
     return od, k
@@ -264,7 +307,7 @@ def grid5(seed: int = 1):
 
 
 def _grid_city(seed, rows, cols, zones, conn_per_zone, shortcuts, spacing, jitter,
-               total_demand, impedance, workload):
+               total_demand, impedance, workload, nonzero=None, demand_factor=1.0):
     rng = np.random.default_rng(seed)
     G = rows * cols
     xy_grid = _jittered_grid(rng, rows, cols, spacing, jitter)
@@ -309,14 +352,29 @@ def _grid_city(seed, rows, cols, zones, conn_per_zone, shortcuts, spacing, jitte
     net = _assemble(zones, zones + G, zones + 1, parts, xy, params)
     if impedance == "sp":
         t0 = freeflow_zone_times(net)
+    elif impedance == "sp-native":
+        t0 = _zone_times_native(net)
     else:  # straight-line proxy at 50 km/h plus the two connectors
         d = np.hypot(xy_zone[:, None, 0] - xy_zone[None, :, 0],
                      xy_zone[:, None, 1] - xy_zone[None, :, 1])
         t0 = d / 50.0 * 60.0 + 2 * CONNECTOR_T0
-    od, k = _gravity(rng, zones, t0, total_demand)
+    od, k = _gravity(rng, zones, t0, total_demand, nonzero=nonzero)
+    counts_total = float(od.sum())
+    nz = int(np.count_nonzero(od))
+    if demand_factor != 1.0:
+        od = od * demand_factor
     dem = Demand.from_dense(od, params=dict(model="gravity Poisson", beta=0.1,
-                                            P_A="LogNormal(5,1)", impedance=impedance,
-                                            target_total=total_demand, scale_k=k))
+                                            P_A="LogNormal(5,1)",
+                                            impedance=("free-flow shortest path"
+                                                       if impedance.startswith("sp")
+                                                       else impedance),
+                                            target_total=total_demand,
+                                            target_nonzero_share=nonzero, scale_k=k,
+                                            count_total=counts_total,
+                                            nonzero_pairs=nz,
+                                            nonzero_share=nz / max(zones * (zones - 1), 1),
+                                            demand_per_count=demand_factor,
+                                            total=float(od.sum())))
     return net, dem
 
 
@@ -330,15 +388,24 @@ def metro(seed: int = 2, scale: float = 1.0, total_demand: float | None = None):
                       "metro" if scale == 1.0 else f"metro@{scale:g}")
 
 
-def regional(seed: int = 3, scale: float = 1.0, total_demand: float | None = None):
-    """Jittered 300x300 grid + 2,000 shortcuts + 4,000 zones (x scale^2)."""
+REGIONAL_NONZERO = 0.30          # share of nonzero OD pairs (Poisson trip counts)
+REGIONAL_DEMAND_PER_COUNT = 2.0 ** -6   # assigned demand per counted trip
+
+
+def regional(seed: int = 3, scale: float = 1.0):
+    """Jittered 300x300 grid + 2,000 shortcuts + 4,000 zones (x scale^2).
+
+    Demand: gravity Poisson trip counts with the free-flow shortest-path
+    impedance, k chosen so that 30% of the off-diagonal OD pairs are
+    expected nonzero (about 115M counted trips at full scale); the assigned
+    demand is counts x 2^-6 (about 1.8M), so the network carries the load of
+    a congested peak hour rather than many times its capacity."""
     rows = cols = max(6, int(round(300 * scale)))
     zones = max(4, int(round(4000 * scale * scale)))
     short = int(round(2000 * scale * scale))
-    if total_demand is None:
-        total_demand = 2.0e6 * scale * scale
-    return _grid_city(seed, rows, cols, zones, 3, short, 0.5, 0.25, total_demand,
-                      "euclid", "regional" if scale == 1.0 else f"regional@{scale:g}")
+    return _grid_city(seed, rows, cols, zones, 3, short, 0.5, 0.25, None,
+                      "sp-native", "regional" if scale == 1.0 else f"regional@{scale:g}",
+                      nonzero=REGIONAL_NONZERO, demand_factor=REGIONAL_DEMAND_PER_COUNT)
 
 
 # ---------------------------------------------------------------------------
