@@ -269,14 +269,22 @@ def bench_batched(args):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     eng.set_profiling(False)
+    path = eng.strategy
+    lanes_min = eng.lanes
+    roof = loading_roofline(acc, path, u.num_nodes, u.num_links, load_traffic("batched"))
+    if args.no_solve:           # profiling runs: the K steps only
+        eng.close()
+        print(json.dumps({"metric": METRIC, "value": None, "unit": "solves/s", "n_gpus": 1,
+                          "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+                          "profiling_run": True, "roofline": roof}))
+        return
     # one cold solve of the whole batch to the target
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     res = solve_batch(eng, TARGET, "batch solve")
     torch.cuda.synchronize()
     batch_s = time.perf_counter() - t1
-    path = eng.strategy
-    lanes_min = eng.lanes
     eng.close()
     iters = [r["iterations"] for r in res]
     assert all(r["converged"] for r in res), "a variant did not converge"
@@ -299,7 +307,6 @@ def bench_batched(args):
             assert r["iterations"] == iters[v], "plain and batched iteration counts differ"
     plain = len(sample) / plain_t
     print(f"plain: {len(sample)} variants, {plain:.3f} solves/s", file=sys.stderr, flush=True)
-    roof = loading_roofline(acc, path, u.num_nodes, u.num_links, load_traffic("batched"))
     sssp = None if args.no_regional else regional_sssp(args)
     cpu = None
     if not args.no_cpu:
@@ -422,6 +429,8 @@ def main():
                     help="batched: variants solved one by one for the plain comparison")
     ap.add_argument("--lanes", type=int, default=None,
                     help="batched: lanes kernel from this many active variants (0: off)")
+    ap.add_argument("--no-solve", action="store_true",
+                    help="batched: time the K steps only (profiling runs)")
     ap.add_argument("--no-regional", action="store_true",
                     help="batched: skip the regional shortest-path measurement")
     ap.add_argument("--regional-iters", type=int, default=2,

@@ -288,3 +288,47 @@ def test_single_source_sssp_entry_points(engine_mod, oracle_lib, hostlib):
         assert net.init[k] == a and net.term[k] == b
         acc = acc + W[k]
     assert acc == od[dest]
+
+
+def test_cli_warm_start_equals_oracle_and_warns_like_the_reference(engine_mod, oracle_lib,
+                                                                   tmp_path):
+    """Warm start from a %.15f flows file (trunk/tap/tap_frankwolfe_pthread.c
+    :549-604,682-690): the run from the file's volumes equals the oracle's
+    solve from the same volumes (correctly rounded x^4 in both) iteration for
+    iteration; costs in the file that are not BPR(volume) draw the
+    reference's "mismatched costs" warning."""
+    from traffic_assignment_amd import tntp
+    net, dem, nf, tf = _files(tmp_path, "metro", 0.3)
+    _run(CLI, ["-i", "10", nf, tf], str(tmp_path / "a.txt"))
+    _, _, va, ca, _ = tntp.read_flows(str(tmp_path / "a.txt"))
+    vals_w, err = _run(CLI, ["-w", str(tmp_path / "a.txt"), "-i", "7", nf, tf],
+                       str(tmp_path / "b.txt"))
+    assert "mismatched costs" not in err                # the file's own costs
+    oracle_lib.lib().ora_set_pow_mode(1)
+    try:
+        o = oracle_lib.fw_solve(net, dem, max_iters=7, nthreads=8, x_init=va)
+    finally:
+        oracle_lib.lib().ora_set_pow_mode(0)
+    _, _, vb, cb, hb = tntp.read_flows(str(tmp_path / "b.txt"))
+    assert vals_w[0] == 7 == o["iterations"]
+    np.testing.assert_allclose(vb, o["x"], rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(cb, o["c"], rtol=1e-12, atol=0)
+    assert abs(hb["relgap"] - o["relgap"]) <= 1e-9 * o["relgap"]
+    # perturb two costs in the file: the warning names link index and nodes
+    lines = open(tmp_path / "a.txt").read().splitlines()
+    rows = [i for i, l in enumerate(lines) if l.startswith("\t") and ":" in l]
+    for j in (3, 17):
+        t = lines[rows[j]].rstrip(";").split("\t")
+        t[-1] = repr(float(t[-1]) + 0.5)
+        lines[rows[j]] = "\t".join(t) + ";"
+    bad = tmp_path / "bad.txt"
+    bad.write_text("\n".join(lines) + "\n")
+    _, err = _run(CLI, ["-w", str(bad), "-i", "1", nf, tf], str(tmp_path / "c.txt"))
+    warn = [l for l in err.splitlines() if l.startswith("warning: mismatched costs")]
+    assert len(warn) == 2
+    assert f"on link 3 from {net.init[3]} to {net.term[3]}" in warn[0]
+    assert f"on link 17 from {net.init[17]} to {net.term[17]}" in warn[1]
+    # the reference's format (its warm start itself is not run: it leaves
+    # the iteration count and relative gap uninitialised)
+    assert re.fullmatch(r"warning: mismatched costs \(delta = 0\.5\d*\) on link 3 from \d+ to \d+",
+                        warn[0]), warn[0]

@@ -8,6 +8,9 @@ Outputs (git-ignored, shipped with the tree):
                                            (tap_frankwolfe / fw_assign /
                                            pape_cuda / TNTP parsers)
   traffic_assignment_amd/tap_frankwolfe_hip  CLI (tap_main.c's flags)
+  traffic_assignment_amd/tap_variants_hip    CLI (tap_mpi_main.c's flags: network
+                                           variants from a modification file,
+                                           solved as one batch)
   traffic_assignment_amd/libtapsynth.so    host-threaded free-flow zone times
                                            for the workload generator
 """
@@ -60,7 +63,8 @@ def build_host():
     _run(["gcc", *C_FLAGS, "-shared", "-I", INC, "-o", lib, *srcs,
           "-L", HERE, "-ltapgpu", "-Wl,-rpath,$ORIGIN", "-lm", "-lpthread"])
     for src, name in (("tap_main_hip.c", "tap_frankwolfe_hip"),
-                      ("tap_greedy_main.c", "tap_greedy_hip")):
+                      ("tap_greedy_main.c", "tap_greedy_hip"),
+                      ("tap_variants_main.c", "tap_variants_hip")):
         main = os.path.join(CSRC, src)
         if os.path.exists(main):
             exe = os.path.join(HERE, name)

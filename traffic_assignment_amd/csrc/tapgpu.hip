@@ -2340,8 +2340,8 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
         hipSuccess || per_cu < 1)
       per_cu = 1;
     c->hbm_grid = std::max(1, ncu * per_cu);
-    // lanes kernel: three bitmaps, then queues from the LDS share of two
-    // workgroups per CU
+    // lanes kernel: three bitmaps, then queues from the LDS share of the
+    // kLanesPerCu workgroups on a CU
     {
       int lds_cu = 0;
       if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
@@ -2350,7 +2350,7 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
       const size_t base = lanes_lds_bytes(c->N, 0);
       c->lanes_ok = base + 256 <= (size_t)lds_max;
       if (c->lanes_ok) {
-        const size_t share = std::min<size_t>((size_t)lds_cu / 2, (size_t)lds_max) - 256;
+        const size_t share = std::min<size_t>((size_t)lds_cu / kLanesPerCu, (size_t)lds_max) - 256;
         const size_t spare = share > base ? share - base : 0;
         c->lanes_qcap_max = (int)std::min<size_t>(kLanesQueueMax, spare / 8) & ~63;
         c->lanes_qcap = c->lanes_qcap_max;

@@ -23,7 +23,8 @@ Workloads (all float64, all seeded, every free parameter recorded in
                  shortest-path time as impedance, scaled so that 30% of the
                  16M OD pairs are nonzero; the assigned demand is counts x
                  2^-6 (peak-hour share; a power of two, so every partial sum
-                 of demand stays exact in FP64).
+                 of demand stays exact in FP64; at reduced scale the power
+                 of two that keeps the load per area).
 * ``upgrades``   32 candidate upgrades on a metro network and a list of
                  budgeted subsets (the batched-variant workload).
 
@@ -361,6 +362,10 @@ def _grid_city(seed, rows, cols, zones, conn_per_zone, shortcuts, spacing, jitte
     od, k = _gravity(rng, zones, t0, total_demand, nonzero=nonzero)
     counts_total = float(od.sum())
     nz = int(np.count_nonzero(od))
+    if demand_factor is None:
+        # a power of two near target / counts: every partial sum of demand
+        # stays exact in FP64
+        demand_factor = 2.0 ** round(np.log2(total_demand / max(counts_total, 1.0)))
     if demand_factor != 1.0:
         od = od * demand_factor
     dem = Demand.from_dense(od, params=dict(model="gravity Poisson", beta=0.1,
@@ -389,7 +394,7 @@ def metro(seed: int = 2, scale: float = 1.0, total_demand: float | None = None):
 
 
 REGIONAL_NONZERO = 0.30          # share of nonzero OD pairs (Poisson trip counts)
-REGIONAL_DEMAND_PER_COUNT = 2.0 ** -6   # assigned demand per counted trip
+REGIONAL_LOAD = 1.8e6            # assigned demand at full scale (x scale^2)
 
 
 def regional(seed: int = 3, scale: float = 1.0):
@@ -397,15 +402,17 @@ def regional(seed: int = 3, scale: float = 1.0):
 
     Demand: gravity Poisson trip counts with the free-flow shortest-path
     impedance, k chosen so that 30% of the off-diagonal OD pairs are
-    expected nonzero (about 115M counted trips at full scale); the assigned
-    demand is counts x 2^-6 (about 1.8M), so the network carries the load of
-    a congested peak hour rather than many times its capacity."""
+    expected nonzero (about 114M counted trips at full scale).  The assigned
+    demand is counts x 2^m, the power of two nearest 1.8M x scale^2 / counts
+    (2^-6 at full scale: about 1.79M), so the network carries the load of a
+    congested peak hour rather than many times its capacity, at every
+    scale."""
     rows = cols = max(6, int(round(300 * scale)))
     zones = max(4, int(round(4000 * scale * scale)))
     short = int(round(2000 * scale * scale))
-    return _grid_city(seed, rows, cols, zones, 3, short, 0.5, 0.25, None,
+    return _grid_city(seed, rows, cols, zones, 3, short, 0.5, 0.25, REGIONAL_LOAD * scale * scale,
                       "sp-native", "regional" if scale == 1.0 else f"regional@{scale:g}",
-                      nonzero=REGIONAL_NONZERO, demand_factor=REGIONAL_DEMAND_PER_COUNT)
+                      nonzero=REGIONAL_NONZERO, demand_factor=None)
 
 
 # ---------------------------------------------------------------------------
