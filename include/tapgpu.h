@@ -117,6 +117,24 @@ int tapgpu_sssp(tapgpu_ctx *ctx, int64_t variant, const int64_t *start_nodes,
                 int64_t num_start, double *dist, int64_t *pred, int64_t ld,
                 int64_t col_base);
 
+/* tapgpu_sssp without allocating device memory, for callers that cannot
+   call hipMalloc (code under HIP graph capture, tight serving loops,
+   harnesses that forbid allocation).  It has no counterpart in the
+   reference.  The start nodes (1-based, num_start of them) and the outputs
+   are device buffers of the caller: d_dist[(node - 1) * ld + i] and
+   d_pred[(node - 1) * ld + i] (link index, -1: none) for node 1..num_nodes
+   and start i, ld >= num_start.  Labels and links are those of
+   tapgpu_sssp.  Its scratch (the start list and, on the labels-in-HBM
+   kernel, 16 bytes per node for each resident workgroup, fewer workgroups
+   when they do not fit) is a 64 MiB array in device memory that each GPU
+   holds for the life of the process; a start list or network beyond it is
+   rejected with TAPGPU_E_LIMIT.  Calls that overlap on one GPU share that
+   array and race.  The context's device must be current; the call waits
+   for its kernels.  It runs the per-origin kernel with labels in LDS where
+   the context uses it, else the one with labels in HBM. */
+int tapgpu_sssp_noalloc(tapgpu_ctx *ctx, int64_t variant, const int64_t *d_start_nodes,
+                        int64_t num_start, double *d_dist, int32_t *d_pred, int64_t ld);
+
 /* All-or-nothing loading with the current costs of variant v into
    volumes[num_links] (overwritten), replacing fw_assign
    (trunk/tap/tap_frankwolfe_cuda.h:26-29) and link_volume_update

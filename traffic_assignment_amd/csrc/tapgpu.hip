@@ -40,6 +40,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2688,6 +2689,145 @@ int tapgpu_sssp(tapgpu_ctx* c, int64_t variant, const int64_t* start_nodes, int6
   (void)hipFree(d_col);
   (void)hipFree(d_gv);
   return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// shortest paths without allocation (tapgpu_sssp_noalloc)
+// ---------------------------------------------------------------------------
+//
+// The per-origin kernels' scratch (the labels-in-HBM kernel's label,
+// predecessor and tail rows, and the 0-based start list of either kernel)
+// lives in this file-scope device array instead of buffers allocated by the
+// context.  Each GPU holds its own copy for the life of the process; host
+// code reaches it only through the address hipGetSymbolAddress returns for
+// the current device (kept per device in g_scratch_ptr).  Every call on a
+// GPU shares that GPU's copy: calls that overlap on one GPU race.
+
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;       // 64 MiB per GPU
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on device `dev` (which is current).
+int scratch_address(tapgpu_ctx* c, int dev, void** out) {
+  if (dev < 0 || dev >= kMaxDevices) return c->fail(TAPGPU_E_ARG, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return c->fail(TAPGPU_E_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return TAPGPU_OK;
+}
+
+// 1-based start nodes -> 0-based list; clears the error flag first; a start
+// node outside 1..N is flagged (code 5) and replaced by node 0.  One block.
+__global__ __launch_bounds__(kBlock) void k_noalloc_starts(const int64_t* starts, int64_t n, int N,
+                                                           int* out, int* err) {
+  if (threadIdx.x < 3) err[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t i = threadIdx.x; i < n; i += kBlock) {
+    const int64_t s = starts[i];
+    if (s < 1 || s > N) {
+      if (atomicCAS(&err[0], 0, 5) == 0) err[1] = (int)i;
+      out[i] = 0;
+    } else {
+      out[i] = (int)(s - 1);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_i32(int* p, long long n, int v) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock)
+    p[i] = v;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tapgpu_sssp_noalloc(tapgpu_ctx* c, int64_t variant, const int64_t* d_start_nodes,
+                        int64_t num_start, double* d_dist, int32_t* d_pred, int64_t ld) {
+  if (!c || num_start < 0 || (num_start > 0 && (!d_start_nodes || !d_dist || !d_pred)))
+    return TAPGPU_E_ARG;
+  int rc = check_variant(c, variant);
+  if (rc) return rc;
+  if (num_start == 0) return TAPGPU_OK;
+  if (ld < num_start)
+    return c->fail(TAPGPU_E_ARG, "ld %lld too small for %lld columns", (long long)ld,
+                   (long long)num_start);
+  if (num_start > INT32_MAX) return c->fail(TAPGPU_E_LIMIT, "too many start nodes");
+  int dev = -1;
+  HIPCHK(c, hipGetDevice(&dev));
+  if (dev != c->device)
+    return c->fail(TAPGPU_E_STATE, "the context belongs to device %d, device %d is current",
+                   c->device, dev);
+  const bool lds = use_lds(c);
+  if (!lds && !c->hbm_ok)
+    return c->fail(TAPGPU_E_LIMIT, "the per-origin kernels do not fit this network");
+  // scratch: the start list, then (labels-in-HBM kernel) rows of 16 B per node
+  const size_t list_bytes = ((size_t)num_start * sizeof(int) + 255) & ~(size_t)255;
+  const size_t row_bytes = 16ull * (size_t)c->N;
+  if (list_bytes + (lds ? 0 : row_bytes) > kNoAllocScratchBytes)
+    return c->fail(TAPGPU_E_LIMIT,
+                   "%lld start nodes on %d nodes exceed the %zu-byte scratch of the no-allocation "
+                   "entry point", (long long)num_start, c->N, kNoAllocScratchBytes);
+  void* base = nullptr;
+  if ((rc = scratch_address(c, dev, &base))) return rc;
+  unsigned char* p = (unsigned char*)base;
+  int* d_src = (int*)p;
+  OriginArgs A = make_origin_args(c, nullptr);
+  A.fww += (size_t)variant * c->E;
+  A.rvw += (size_t)variant * c->E;
+  A.src_list = d_src;
+  A.out_dist = d_dist;
+  A.out_pred = d_pred;
+  A.ld = ld;
+  int rows = 0;
+  if (!lds) {
+    rows = (int)std::min<size_t>((size_t)c->hbm_grid, (kNoAllocScratchBytes - list_bytes) / row_bytes);
+    const size_t rN = (size_t)rows * c->N;
+    A.h_dist = (unsigned long long*)(p + list_bytes);
+    A.h_pred = (int*)(p + list_bytes + 8 * rN);
+    A.h_ptail = (int*)(p + list_bytes + 12 * rN);
+  }
+  hipLaunchKernelGGL(k_noalloc_starts, dim3(1), dim3(kBlock), 0, c->stream, d_start_nodes, num_start,
+                     c->N, d_src, c->err_dev);
+  // predecessors start at -1, so a failed search hands back no stale link
+  const long long npred = (long long)c->N * ld;
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(npred / kWave + 1)), dim3(kBlock), 0, c->stream,
+                     d_pred, npred, -1);
+  if (lds)
+    hipLaunchKernelGGL(k_origin_lds, dim3((unsigned)num_start, 1), dim3(kLdsBlock), c->lds_bytes,
+                       c->stream, A, 0);
+  else
+    hipLaunchKernelGGL(k_origin_hbm, dim3((unsigned)std::min<int64_t>(num_start, rows)),
+                       dim3(kHbmBlock), c->hbm_lds, c->stream, A, 0, (int)num_start);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(c->h_int, c->err_dev, sizeof(int) * 3, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int* herr = c->h_int;
+  if (herr[0] == 5)
+    return c->fail(TAPGPU_E_ARG, "start node %d of the list is out of range", herr[1]);
+  if (herr[0] == 2)
+    return c->fail(TAPGPU_E_LIMIT, "label-correcting rounds exceeded from origin %d (negative costs?)",
+                   herr[1] + 1);
+  if (herr[0] == 3)
+    return c->fail(TAPGPU_E_LIMIT, "predecessor cycle in the tree of origin %d", herr[1] + 1);
+  if (herr[0] != 0)
+    return c->fail(TAPGPU_E_LIMIT, "shortest-path kernel reported error %d at origin %d", herr[0],
+                   herr[1] + 1);
+  return TAPGPU_OK;
 }
 
 int tapgpu_aon(tapgpu_ctx* c, int64_t variant, double* volumes) {
