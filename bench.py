@@ -322,7 +322,9 @@ def bench_batched(args):
         "config": {"workload": f"batched: {len(subsets)} upgrade subsets of metro (11,000 nodes / "
                                "43,600 links / 1,000 zones), 32 candidate upgrades, budget 40",
                    "step": f"one Frank-Wolfe iteration of all {len(subsets)} variants",
-                   "target_relgap": TARGET, "engine_path": path, "lanes_min_active": lanes_min,
+                   "target_relgap": TARGET,
+                   "engine_path": f"lanes kernel while >= {lanes_min} variants run, then {path}",
+                   "lanes_min_active": lanes_min,
                    "iterations_min_median_max": [min(iters), int(np.median(iters)), max(iters)],
                    "parallelism": "1 GPU"},
         "batch_solve_s": round(batch_s, 3),
@@ -437,7 +439,7 @@ def main():
                     help="batched: timed FW iterations of the regional network")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--strategy", default="auto", choices=["auto", "sweeps", "lds", "hbm"])
-    ap.add_argument("--chase", type=int, default=0,
+    ap.add_argument("--chase", type=int, default=-1,
                     help="LDS path: chained expansions per bitmap-scan round (-1: tuned)")
     ap.add_argument("--delta", type=float, default=-1.0,
                     help="HBM path: bucket width in mean free-flow times (-1: tuned)")

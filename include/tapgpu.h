@@ -171,6 +171,22 @@ int tapgpu_fw_run(tapgpu_ctx *ctx, int max_iters, double target_relgap,
 /* Volumes and costs of variant v (link order of the context's network). */
 int tapgpu_get_flows(tapgpu_ctx *ctx, int64_t variant, double *volumes, double *costs);
 
+/* Beckmann objective (the sum of link cost integrals of
+   links_objective_function, trunk/tap/tap_functions.c:112-152) and VHT
+   (sum x * cost) of variant v at its current volumes, compensated sums in a
+   fixed order.  (fw_run's per-iteration objective is the one before that
+   iteration's step.) */
+int tapgpu_get_totals(tapgpu_ctx *ctx, int64_t variant, double *objective, double *vht);
+
+/* Replace variant v's link capacities and/or free-flow times (num_links
+   host values each; NULL keeps the current ones), with the capacity rule of
+   tapgpu_create (<= 0: the variant's smallest positive capacity).  No
+   counterpart in the reference: it lets one context serve many link
+   parameter sets (gradient screening) without rebuilding.  The volumes no
+   longer match the links: tapgpu_fw_init must come next. */
+int tapgpu_set_links(tapgpu_ctx *ctx, int64_t variant, const double *capacity,
+                     const double *free_flow_time);
+
 /* Copy up to cap per-iteration records of variant v; *count = number held. */
 int tapgpu_get_trace(tapgpu_ctx *ctx, int64_t variant, tapgpu_iter *trace, int cap,
                      int *count);
@@ -219,11 +235,19 @@ typedef struct {
   int64_t origin_split;         /* origins per variant co-run by the HBM     */
                                 /*   kernel beside the LDS kernel            */
   int64_t origin_deferred;      /* queued nodes kept for a later bucket      */
+  int64_t origin_block;         /* LDS path: threads per workgroup           */
+  int64_t origin_queue;         /* frontier queue entries of the kernel used */
   /* lanes kernel (batched loading, one variant per wavefront lane) */
   int64_t lanes_launches;       /* launches                                  */
   int64_t lanes_tasks;          /* (origin, 64-variant group) workgroups     */
   int64_t lanes_pred_scans;     /* predecessor selections (lanes)            */
   int64_t lanes_fallbacks;      /* loadings redone by a per-origin kernel    */
+  double lanes_delta;           /* its bucket width in mean free-flow times  */
+  int64_t graph_launches;       /* iterations replayed from captured graphs  */
+  /* multi-origin kernel (strategy 4) */
+  int64_t multi_launches;       /* launches                                  */
+  int64_t multi_fallbacks;      /* loadings redone by the labels-in-HBM kernel */
+  int64_t multi_k;              /* origins per workgroup                     */
 } tapgpu_stats;
 
 /* Counters and wall times of the last tapgpu_fw_run (or of the calls since
@@ -238,27 +262,48 @@ int tapgpu_set_profiling(tapgpu_ctx *ctx, int on);
    origin per workgroup with labels in LDS (networks whose labels fit a
    workgroup's LDS: about 13k nodes, fewer than 65535 links), 3 = one origin
    per workgroup with labels in HBM (frontier bitmaps in LDS: up to about
-   430k nodes).  tapgpu_get_strategy returns the one in use (1, 2 or 3). */
+   430k nodes), 4 = K neighbouring origins per workgroup with their labels
+   side by side in HBM (loadings; shortest-path-only calls and loadings with
+   zero-cost level ties use 3).  tapgpu_get_strategy returns the one in use
+   (1, 2, 3 or 4). */
 int tapgpu_set_strategy(tapgpu_ctx *ctx, int strategy);
+
+/* Schedule of strategy 4: origins per workgroup (4, 8 or 16) and the bucket
+   width of its label-ordered rounds in mean free-flow link times (0: none).
+   -1 (the default for either): tuned on the next loading by timing each
+   candidate on the real launch.  Results do not depend on them. */
+int tapgpu_set_multi(tapgpu_ctx *ctx, int origins, double delta_mult);
+int tapgpu_get_multi(const tapgpu_ctx *ctx, int *origins, double *delta_mult);
 
 /* Worklist schedule of the one-origin-per-workgroup path (strategy 2),
    rounds that scan the frontier bitmap (see tapgpu_set_origin_queue): how
    many labels a thread lowers and then expands itself, in a chain, within
-   one round (0, the default: every lowered node waits for the next round).
-   -1: tuned on the next loading by timing each candidate
+   one round (0: every lowered node waits for the next round).  -1 (the
+   default): tuned on the next loading by timing each candidate
    (0, 2, 4, 8, 16, 32) on the real launch.  Results do not depend on it. */
 int tapgpu_set_origin_chase(tapgpu_ctx *ctx, int chase);
 int tapgpu_get_origin_chase(const tapgpu_ctx *ctx);
 
 /* Frontier queues of the one-origin-per-workgroup paths (strategies 2 and
-   3) and of the lanes kernel: a round whose frontier (the nodes lowered or kept in the round
-   before) has at most this many nodes takes them from an LDS list; a
-   larger frontier is found by scanning its bitmap.  -1 (the default): the
-   most each kernel's spare LDS holds (up to 4096); 0: bitmap scans only;
-   larger values are clamped to that most.  tapgpu_get_origin_queue returns
-   the value of the kernel in use.  Results do not depend on it. */
+   3) and of the lanes kernel: a round whose frontier (the nodes lowered or
+   kept in the round before) has at most this many nodes takes them from an
+   LDS list; a larger frontier is found by scanning its bitmap.  -1 (the
+   default): the most each kernel's spare LDS holds (up to 4096), except on
+   the LDS path, where the next loading picks it with the workgroup size
+   (the most, 1024 or 256: a shorter queue takes less LDS, so more
+   workgroups share a CU); 0: bitmap scans only; larger values are clamped
+   to that most.  tapgpu_get_origin_queue returns the value of the kernel in
+   use.  Results do not depend on it. */
 int tapgpu_set_origin_queue(tapgpu_ctx *ctx, int entries);
 int tapgpu_get_origin_queue(const tapgpu_ctx *ctx);
+
+/* Workgroup size of the one-origin-per-workgroup LDS path (strategy 2):
+   1024, 512 or 256 threads (a size holds networks of up to 16 nodes per
+   thread).  -1 (the default): tuned on the next loading, with the queue
+   size, by timing every allowed pair on the real launch; get returns -1
+   until then.  Results do not depend on it. */
+int tapgpu_set_origin_block(tapgpu_ctx *ctx, int threads);
+int tapgpu_get_origin_block(const tapgpu_ctx *ctx);
 
 /* Worklist schedule of the labels-in-HBM path (strategy 3): bucket width of
    its label-ordered rounds, as a multiple of the mean free-flow link time
@@ -289,6 +334,25 @@ int tapgpu_get_strategy(const tapgpu_ctx *ctx);
    Results do not depend on it (same trees, same volumes bit for bit). */
 int tapgpu_set_lanes(tapgpu_ctx *ctx, int min_active);
 int tapgpu_get_lanes(const tapgpu_ctx *ctx);
+
+/* Worklist schedule of the lanes kernel: bucket width of its label-ordered
+   rounds (on the smallest label over a node's lanes), as a multiple of the
+   mean free-flow link time (0: no buckets).  -1 (the default): tuned on the
+   first loading by timing each candidate (0, 1, 2, 4, 8) on one wave of
+   workgroups.  Results do not depend on it. */
+int tapgpu_set_lanes_delta(tapgpu_ctx *ctx, double mult);
+double tapgpu_get_lanes_delta(const tapgpu_ctx *ctx);
+
+/* Launch form of tapgpu_fw_run's iterations (no counterpart in the
+   reference, whose steps are host loops).  on = 1 (the default): once the
+   per-origin schedule is tuned, each iteration (opening kernel, loading,
+   both line-search windows, step, readback) is one replay of a captured HIP
+   graph; with the lanes kernel the link passes are.  on = 0: the same
+   launches issued one by one.  Profiling (tapgpu_set_profiling) issues them
+   one by one too, so the events around the loading are recorded.  Results
+   do not depend on it. */
+int tapgpu_set_graphs(tapgpu_ctx *ctx, int on);
+int tapgpu_get_graphs(tapgpu_ctx *ctx, int *on);
 
 /* --- greedy step-cost assignment ------------------------------------------ */
 

@@ -10,6 +10,8 @@ solver and loaded into the engine as warm-start volumes).
 """
 from __future__ import annotations
 
+import dataclasses
+
 import numpy as np
 import pytest
 import torch
@@ -212,3 +214,57 @@ def test_autograd_function(engine_mod):
     obj2, vht2, _ = equilibrium(net, dem, cap2, t0.detach(), target_relgap=1e-5)
     vht2.backward()
     assert torch.isfinite(cap2.grad).all() and abs(obj2.item() - Zs) <= 1e-4 * Zs
+
+
+def test_autograd_reuses_engine(engine_mod):
+    """Calls on one network and demand share one engine (capacities and
+    free-flow times replaced per call): every call's volumes equal a freshly
+    built engine's solve bit for bit, objective and VHT the oracle's sums,
+    the gradient the fresh engine's to 1e-8; a demand changed in place is
+    seen (the engine is rebuilt)."""
+    from traffic_assignment_amd.python_extension import autograd as A
+    import oracle
+    print("case autograd engine reuse", flush=True)
+    net, dem = network("metro", 0.3)
+    rng = np.random.default_rng(5)
+    A._CACHE.close()
+    b0 = A._CACHE.builds
+    for k in range(3):
+        capk = net.capacity * rng.uniform(0.8, 1.2, net.num_links)
+        t0k = net.fftime * rng.uniform(0.9, 1.1, net.num_links)
+        cap = torch.tensor(capk, dtype=torch.float64, device="cuda", requires_grad=True)
+        t0 = torch.tensor(t0k, dtype=torch.float64, device="cuda", requires_grad=True)
+        obj, vht, x = A.equilibrium(net, dem, cap, t0, target_relgap=1e-4, eps=1e-3)
+        vht.backward()
+        assert A._CACHE.builds == b0 + 1
+        with engine_mod.TapEngine(net, dem, num_variants=1, capacity=capk[None, :],
+                                  fftime=t0k[None, :]) as eng:
+            eng.fw_init()
+            eng.fw_run(target_relgap=1e-4)
+            xf, cf = eng.flows()
+            g = eng.gradient(eps=1e-3)
+        assert np.array_equal(x.cpu().numpy(), xf)
+        assert abs(vht.item() - float(np.dot(xf, cf))) <= 1e-12 * vht.item()
+        netk = dataclasses.replace(net, capacity=capk, fftime=t0k)
+        Z = oracle.objective(netk, xf)
+        assert abs(obj.item() - Z) <= 1e-12 * abs(Z)
+        # the zone order of a context (and with it the order of the
+        # cycle sums in conjugate gradients) comes from the free-flow times
+        # it was built with, so the two gradients agree to the solve's
+        # tolerance, not bit for bit
+        ga = cap.grad.cpu().numpy()
+        np.testing.assert_allclose(ga, g["dvht_dcap"], rtol=1e-8,
+                                   atol=1e-10 * np.abs(g["dvht_dcap"]).max())
+    dem.demand[0] += 1.0
+    try:
+        cap = torch.tensor(net.capacity, dtype=torch.float64, device="cuda")
+        t0 = torch.tensor(net.fftime, dtype=torch.float64, device="cuda")
+        _, _, x = A.equilibrium(net, dem, cap, t0, target_relgap=1e-4)
+        assert A._CACHE.builds == b0 + 2
+        with engine_mod.TapEngine(net, dem) as eng:
+            eng.fw_init()
+            eng.fw_run(target_relgap=1e-4)
+            assert np.array_equal(x.cpu().numpy(), eng.flows()[0])
+    finally:
+        dem.demand[0] -= 1.0
+        A._CACHE.close()

@@ -169,16 +169,43 @@ def test_pape_cuda_fw_assign_link_volume_shims(engine_mod, oracle_lib, hostlib):
             assert dist[v * Z + s] == od[v, s - 1]
     assert sum(pred[v * Z + s] == op[v, s - 1] for s in range(1, Z + 1) for v in range(1, N1)) \
         == Z * (N1 - 1)
+    from traffic_assignment_amd import synth
+    # start nodes not 1..Z in order (the shim lays the columns out itself),
+    # at other costs (uploaded again on the cached topology)
+    Wb = np.ascontiguousarray(Wa * rng.uniform(1, 1.5, Wa.size))
+    rev = np.ascontiguousarray(starts[::-1])
+    dist2 = np.zeros(N1 * (Z + 1)); pred2 = np.zeros(N1 * (Z + 1), dtype=np.int64)
+    hostlib.pape_cuda(Va.ctypes.data_as(lp), Ea.ctypes.data_as(lp), Wb.ctypes.data_as(C.POINTER(C.c_double)),
+                      C.c_long(N1), C.c_long(E), rev.ctypes.data_as(lp), C.c_long(rev.size),
+                      C.c_long(net.first_thru_node), dist2.ctypes.data_as(C.POINTER(C.c_double)),
+                      pred2.ctypes.data_as(lp))
+    od2, op2 = oracle_lib.sssp_all(net, Wb)
+    S = rev.size
+    for s in rev:
+        got = np.array([dist2[v * S + s] for v in range(1, N1)])
+        assert np.array_equal(got, od2[1:, s - 1])
     # fw_assign: volumes of the oracle's all-or-nothing step
     cnet, keep = _c_net(net)
     cdem, keep2 = _c_demands(dem)
     y = np.zeros(E)
     scratch_d = np.zeros(N1 * (Z + 1)); scratch_p = np.zeros(N1 * (Z + 1), dtype=np.int64)
-    for _ in range(2):                         # second call reuses the cached context
+
+    def assign(W):
         hostlib.fw_assign(C.byref(cnet), cdem, Va.ctypes.data_as(lp), Ea.ctypes.data_as(lp),
-                          Wa.ctypes.data_as(C.POINTER(C.c_double)), y.ctypes.data_as(C.POINTER(C.c_double)),
+                          W.ctypes.data_as(C.POINTER(C.c_double)), y.ctypes.data_as(C.POINTER(C.c_double)),
                           scratch_d.ctypes.data_as(C.POINTER(C.c_double)), scratch_p.ctypes.data_as(lp))
-        assert np.array_equal(y, oracle_lib.fw_assign(net, dem, Wa, nthreads=8))
+        return y.copy()
+
+    for W in (Wa, Wb):                         # second call reuses the cached context
+        assert np.array_equal(assign(W), oracle_lib.fw_assign(net, dem, W, nthreads=8))
+    # demand changed in place, same pointers: the shim must see it
+    o = int(np.argmax(np.diff(dem.start)))
+    keep2[o][0].demand += 1000.0
+    dem2 = synth.Demand(dem.num_zones, dem.start, dem.dest, dem.demand.copy())
+    dem2.demand[int(dem.start[o])] += 1000.0
+    assert np.array_equal(assign(Wb), oracle_lib.fw_assign(net, dem2, Wb, nthreads=8))
+    keep2[o][0].demand -= 1000.0
+    assert np.array_equal(assign(Wa), oracle_lib.fw_assign(net, dem, Wa, nthreads=8))
     # link_volume_update from the oracle's own predecessor links
     hostlib.link_volume_data_setup(keep, C.c_long(E), C.c_long(Z), cdem)
     plink = np.zeros(N1 * (Z + 1), dtype=np.int64)

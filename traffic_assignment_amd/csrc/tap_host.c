@@ -10,8 +10,8 @@
  *
  * The engine context (graph, link records and demand resident in HBM) is
  * created once and cached; the shims that receive raw arrays every call
- * (fw_assign, pape_cuda) reuse it while the topology and demand are
- * unchanged and only upload the new link costs.
+ * (fw_assign, pape_cuda, sssp_*) compare them with the arrays the context
+ * was built from (one read pass) and only upload the new link costs.
  */
 #include <errno.h>
 #include <float.h>
@@ -39,19 +39,8 @@ static void die(const char *what, tapgpu_ctx *ctx, int rc)
   exit(1);
 }
 
-/* multiply-xorshift hash over 64-bit words (every input array here is made
-   of 8-byte elements): identifies the topology / demand a cached context was
-   built from */
-static uint64_t fnv(uint64_t h, const void *p, size_t n)
-{
-  const uint64_t *w = p;
-  for (size_t i = 0; i < n / 8; i++) {
-    h ^= w[i] + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
-    h *= 0xBF58476D1CE4E5B9ull;
-    h ^= h >> 31;
-  }
-  return h;
-}
+/* pape_cuda's predecessor array (long) is handed to the engine as int64_t */
+_Static_assert(sizeof(long) == sizeof(int64_t), "LP64 expected");
 
 /* --- network + demand -> engine arrays --------------------------------------- */
 
@@ -113,52 +102,42 @@ static int make_input(const net_data_t *net, demand_data_t *const *demands, engi
   return 0;
 }
 
-static uint64_t input_key(const engine_input *in)
-{
-  uint64_t h = 1469598103934665603ull;
-  h = fnv(h, &in->net.num_nodes, sizeof(int64_t));
-  h = fnv(h, &in->net.num_zones, sizeof(int64_t));
-  h = fnv(h, &in->net.first_thru_node, sizeof(int64_t));
-  h = fnv(h, in->init, sizeof(int64_t) * in->net.num_links);
-  h = fnv(h, in->term, sizeof(int64_t) * in->net.num_links);
-  h = fnv(h, in->cap, sizeof(double) * in->net.num_links);
-  h = fnv(h, in->t0, sizeof(double) * in->net.num_links);
-  h = fnv(h, in->B, sizeof(double) * in->net.num_links);
-  h = fnv(h, in->pw, sizeof(double) * in->net.num_links);
-  h = fnv(h, in->od_start, sizeof(int64_t) * (in->dem.num_zones + 1));
-  h = fnv(h, in->od_dest, sizeof(int64_t) * in->od_start[in->dem.num_zones]);
-  h = fnv(h, in->od_val, sizeof(double) * in->od_start[in->dem.num_zones]);
-  return h;
-}
-
-/* one cached context per shim family */
-typedef struct {
-  tapgpu_ctx *ctx;
-  uint64_t key;
-} cached_ctx;
-
-static int cached_get(cached_ctx *c, const engine_input *in, tapgpu_ctx **out)
-{
-  const uint64_t key = input_key(in);
-  if (c->ctx && c->key == key) {
-    *out = c->ctx;
-    return TAPGPU_OK;
-  }
-  if (c->ctx) {
-    tapgpu_destroy(c->ctx);
-    c->ctx = NULL;
-  }
-  int rc = tapgpu_create(&c->ctx, &in->net, &in->dem);
-  if (rc)
-    return rc;
-  c->key = key;
-  *out = c->ctx;
-  return TAPGPU_OK;
-}
-
 /* --- fw_assign ------------------------------------------------------------------ */
 
-static cached_ctx g_assign;
+/* The context fw_assign reuses, with the arrays it was built from.  A call
+   whose network and demand equal them (one read pass: no allocation, no
+   rebuild) only uploads the new costs. */
+static struct {
+  tapgpu_ctx *ctx;
+  engine_input in;
+} g_assign;
+
+static int same_input(const engine_input *in, const net_data_t *net, demand_data_t *const *demands)
+{
+  const long E = net->num_links, Z = net->num_zones;
+  if (!in->init || in->net.num_links != E || in->net.num_zones != Z ||
+      in->net.num_nodes != net->num_nodes || in->net.first_thru_node != net->first_thru_node)
+    return 0;
+  for (long k = 0; k < E; k++) {
+    const link_data_t *l = &net->links[k];
+    if (in->init[k] != l->init_node || in->term[k] != l->term_node ||
+        memcmp(&in->cap[k], &l->capacity, sizeof(double)) ||
+        memcmp(&in->t0[k], &l->free_flow_time, sizeof(double)) ||
+        memcmp(&in->B[k], &l->B, sizeof(double)) || memcmp(&in->pw[k], &l->power, sizeof(double)))
+      return 0;
+  }
+  const int64_t nnz = in->od_start[Z];
+  int64_t j = 0;
+  for (long o = 1; o <= Z; o++) {
+    for (long i = 0; demands && demands[o] && demands[o][i].dest != 0; i++, j++) {
+      if (j >= nnz || in->od_dest[j] != demands[o][i].dest ||
+          memcmp(&in->od_val[j], &demands[o][i].demand, sizeof(double)))
+        return 0;
+    }
+    if (in->od_start[o] != j) return 0;
+  }
+  return 1;
+}
 
 /* All-or-nothing assignment of demands at costs Wa into link_volumes
    (zeroed first).  Va/Ea must be the CSR of net->links (link index ==
@@ -168,14 +147,19 @@ void fw_assign(net_data_t *net, demand_data_t *demands[], long Va[], long Ea[], 
                double link_volumes[], double dist[], long predlink[])
 {
   (void)Va; (void)Ea; (void)dist; (void)predlink;
-  engine_input in;
-  if (make_input(net, demands, &in)) {
-    fprintf(stderr, "malloc failed in fw_assign\n");
-    exit(1);
+  int rc;
+  if (!g_assign.ctx || !same_input(&g_assign.in, net, demands)) {
+    if (g_assign.ctx) tapgpu_destroy(g_assign.ctx);
+    g_assign.ctx = NULL;
+    free_input(&g_assign.in);
+    if (make_input(net, demands, &g_assign.in)) {
+      fprintf(stderr, "malloc failed in fw_assign\n");
+      exit(1);
+    }
+    if ((rc = tapgpu_create(&g_assign.ctx, &g_assign.in.net, &g_assign.in.dem)))
+      die("tapgpu_create", NULL, rc);
   }
-  tapgpu_ctx *ctx = NULL;
-  int rc = cached_get(&g_assign, &in, &ctx);
-  if (rc) die("tapgpu_create", NULL, rc);
+  tapgpu_ctx *ctx = g_assign.ctx;
   if ((rc = tapgpu_set_costs(ctx, 0, Wa))) die("tapgpu_set_costs", ctx, rc);
   rc = tapgpu_aon(ctx, 0, link_volumes);
   if (rc == TAPGPU_E_NOPATH) {
@@ -183,77 +167,6 @@ void fw_assign(net_data_t *net, demand_data_t *demands[], long Va[], long Ea[], 
     exit(1);
   }
   if (rc) die("tapgpu_aon", ctx, rc);
-  free_input(&in);
-}
-
-/* --- pape_cuda ---------------------------------------------------------------- */
-
-static cached_ctx g_pape;
-
-/* All-origin shortest paths over the CSR (Va, Ea, Wa) from start_nodes; the
-   reference's output layout: entry [v * num_start_nodes + s] for start node
-   s and node v (0 <= v < num_nodes). */
-void pape_cuda(long Va[], long Ea[], double Wa[], long num_nodes, long num_edges,
-               long start_nodes[], long num_start_nodes, long first_thru_node,
-               double *distances, long *predecessors)
-{
-  engine_input in;
-  memset(&in, 0, sizeof in);
-  const long N = num_nodes - 1;                  /* node ids 1..N, row 0 unused */
-  in.init = malloc(sizeof(int64_t) * (num_edges + 1));
-  in.term = malloc(sizeof(int64_t) * (num_edges + 1));
-  in.cap = malloc(sizeof(double) * (num_edges + 1));
-  in.t0 = malloc(sizeof(double) * (num_edges + 1));
-  in.B = calloc((size_t)num_edges + 1, sizeof(double));
-  in.pw = malloc(sizeof(double) * (num_edges + 1));
-  in.od_start = calloc(1, sizeof(int64_t));
-  in.od_dest = malloc(sizeof(int64_t));
-  in.od_val = malloc(sizeof(double));
-  if (!in.init || !in.term || !in.cap || !in.t0 || !in.B || !in.pw || !in.od_start) {
-    fprintf(stderr, "malloc failed in pape_cuda\n");
-    exit(1);
-  }
-  for (long v = 0; v < num_nodes; v++)
-    for (long e = Va[v]; e < Va[v + 1]; e++) {
-      in.init[e] = v;
-      in.term[e] = Ea[e];
-    }
-  for (long e = 0; e < num_edges; e++) {
-    in.cap[e] = 1.0;
-    in.t0[e] = 0.0;                              /* topology only in the key */
-    in.pw[e] = 1.0;
-  }
-  if (num_edges > 0 && Va[0] != Va[1]) {
-    fprintf(stderr, "ERROR: pape_cuda: node 0 has links (node ids are 1-based)\n");
-    exit(1);
-  }
-  in.net = (tapgpu_network){0, N, first_thru_node, num_edges, in.init, in.term, in.cap,
-                            in.t0, in.B, in.pw};
-  in.dem = (tapgpu_demand){0, in.od_start, in.od_dest, in.od_val};
-  tapgpu_ctx *ctx = NULL;
-  int rc = cached_get(&g_pape, &in, &ctx);
-  if (rc) die("tapgpu_create", NULL, rc);
-  if ((rc = tapgpu_set_costs(ctx, 0, Wa))) die("tapgpu_set_costs", ctx, rc);
-  int64_t *starts = malloc(sizeof(int64_t) * (num_start_nodes + 1));
-  double *d = malloc(sizeof(double) * (size_t)num_nodes * (num_start_nodes + 1));
-  int64_t *p = malloc(sizeof(int64_t) * (size_t)num_nodes * (num_start_nodes + 1));
-  if (!starts || !d || !p) {
-    fprintf(stderr, "malloc failed in pape_cuda\n");
-    exit(1);
-  }
-  for (long i = 0; i < num_start_nodes; i++)
-    starts[i] = start_nodes[i];
-  rc = tapgpu_sssp(ctx, 0, starts, num_start_nodes, d, p, num_start_nodes, 0);
-  if (rc) die("tapgpu_sssp", ctx, rc);
-  /* column = start node id, as the reference kernel indexes it */
-  for (long v = 0; v < num_nodes; v++)
-    for (long i = 0; i < num_start_nodes; i++) {
-      const size_t o = (size_t)v * num_start_nodes + start_nodes[i];
-      distances[o] = d[(size_t)v * num_start_nodes + i];
-      predecessors[o] = (long)p[(size_t)v * num_start_nodes + i];
-    }
-  free(starts); free(d); free(p);
-  free_input(&in);
 }
 
 /* --- single-source shortest paths (sssp_pape.h) ------------------------------- */
@@ -400,6 +313,64 @@ static void sssp_run(long Va[], long Ea[], double Wa[], long num_nodes, long num
     }
     free(d); free(p);
   }
+  pthread_mutex_unlock(&g_sssp_lock);
+}
+
+/* --- pape_cuda ---------------------------------------------------------------- */
+
+/* All-origin shortest paths over the CSR (Va, Ea, Wa) from start_nodes; the
+   reference's output layout: entry [v * num_start_nodes + s] for start node
+   s and node v (0 <= v < num_nodes).  The context is the shortest-path
+   family's (topology compared, costs uploaded when they change); with start
+   nodes 1..num_start_nodes (fw_assign's) the engine writes that layout
+   directly. */
+void pape_cuda(long Va[], long Ea[], double Wa[], long num_nodes, long num_edges,
+               long start_nodes[], long num_start_nodes, long first_thru_node,
+               double *distances, long *predecessors)
+{
+  pthread_mutex_lock(&g_sssp_lock);
+  sssp_context(Va, Ea, num_nodes, num_edges, first_thru_node);
+  tapgpu_ctx *ctx = g_sssp.ctx;
+  int rc;
+  if (!g_sssp.costs_set || memcmp(g_sssp.Wa, Wa, sizeof(double) * (size_t)num_edges) != 0) {
+    if ((rc = tapgpu_set_costs(ctx, 0, Wa))) die("tapgpu_set_costs", ctx, rc);
+    memcpy(g_sssp.Wa, Wa, sizeof(double) * (size_t)num_edges);
+    g_sssp.costs_set = 1;
+    g_sssp.zones = 0;
+  }
+  int64_t *starts = malloc(sizeof(int64_t) * (num_start_nodes + 1));
+  if (!starts) {
+    fprintf(stderr, "malloc failed in pape_cuda\n");
+    exit(1);
+  }
+  int dense = 1;
+  for (long i = 0; i < num_start_nodes; i++) {
+    starts[i] = start_nodes[i];
+    dense = dense && start_nodes[i] == i + 1;
+  }
+  if (dense) {
+    /* column = start node id: col_base 1, row stride num_start_nodes */
+    rc = tapgpu_sssp(ctx, 0, starts, num_start_nodes, distances, (int64_t *)predecessors,
+                     num_start_nodes, 1);
+    if (rc) die("tapgpu_sssp", ctx, rc);
+  } else {
+    double *d = malloc(sizeof(double) * (size_t)num_nodes * (num_start_nodes + 1));
+    int64_t *p = malloc(sizeof(int64_t) * (size_t)num_nodes * (num_start_nodes + 1));
+    if (!d || !p) {
+      fprintf(stderr, "malloc failed in pape_cuda\n");
+      exit(1);
+    }
+    rc = tapgpu_sssp(ctx, 0, starts, num_start_nodes, d, p, num_start_nodes, 0);
+    if (rc) die("tapgpu_sssp", ctx, rc);
+    for (long v = 0; v < num_nodes; v++)
+      for (long i = 0; i < num_start_nodes; i++) {
+        const size_t o = (size_t)v * num_start_nodes + start_nodes[i];
+        distances[o] = d[(size_t)v * num_start_nodes + i];
+        predecessors[o] = (long)p[(size_t)v * num_start_nodes + i];
+      }
+    free(d); free(p);
+  }
+  free(starts);
   pthread_mutex_unlock(&g_sssp_lock);
 }
 

@@ -448,10 +448,15 @@ __global__ __launch_bounds__(kBlock) void k_aon_reduce(int E, int Gv, int v0, in
 // while the level decreases, so the tree is acyclic.  Where the shortest
 // path is unique this is the reference's predecessor.
 
+// Workgroup sizes of the LDS kernel; a size serves networks of up to
+// kLdsPerThread nodes per thread (the per-thread register arrays of the
+// predecessor phase).  1024 by default; the schedule tuner may pick a
+// smaller one where it fits.
 constexpr int kLdsBlock = 1024;
+constexpr int kLdsBlocks[] = {1024, 512, 256};
 constexpr int kDefaultChase = 4;               // before tuning / shortest-path-only calls
-constexpr int kLdsMaxNodes = 16384;            // per-thread register arrays below
-constexpr int kLdsPerThread = kLdsMaxNodes / kLdsBlock;
+constexpr int kLdsPerThread = 16;
+constexpr int kLdsMaxNodes = kLdsPerThread * kLdsBlock;
 #ifndef TAPGPU_LDS_QUEUE_MAX
 #define TAPGPU_LDS_QUEUE_MAX 4096
 #endif
@@ -562,7 +567,8 @@ __device__ bool attach_level_ties(const OriginArgs& A, const double* __restrict_
   }
 }
 
-__global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load) {
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_origin_lds(OriginArgs A, int load) {
   extern __shared__ unsigned long long sm[];
   const int N = A.N;
   const int W = (N + 63) >> 6;
@@ -588,9 +594,9 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
   const double* __restrict__ rw = A.rvw + (size_t)var * A.E;
   const unsigned long long kMax = as_b(DBL_MAX);
 
-  for (int v = tid; v < N; v += kLdsBlock) dist[v] = kMax;
-  for (int v = tid; v <= N; v += kLdsBlock) offs[v] = (unsigned short)A.fw_off[v];
-  for (int w = tid; w < 3 * W; w += kLdsBlock) bmA[w] = 0;
+  for (int v = tid; v < N; v += BLOCK) dist[v] = kMax;
+  for (int v = tid; v <= N; v += BLOCK) offs[v] = (unsigned short)A.fw_off[v];
+  for (int w = tid; w < 3 * W; w += BLOCK) bmA[w] = 0;
   if (tid < 3) s_qn[tid] = tid == 0 ? 1 : 0;
   if (tid < 8) s_cnt[tid] = 0;
   __syncthreads();
@@ -654,7 +660,7 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
     const int u0 = j0 < A.qcap ? qr[j0] : 0;      // read beside the count
     const int qn = s_qn[(r - 1) % 3];
     if (qn <= A.qcap) {
-      for (int idx = tid; idx / kGroup < qn; idx += kLdsBlock) {
+      for (int idx = tid; idx / kGroup < qn; idx += BLOCK) {
         const int u = idx == tid ? u0 : qr[idx / kGroup];
         const int g = idx % kGroup;
         const double du = as_d(dist[u]);
@@ -663,7 +669,7 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
         for (int e = eb + g; e < ee; e += kGroup) relax(du, A.fw_dst[e], fw[e], true);
       }
     } else {
-      for (int slot = tid; slot < 4 * W; slot += kLdsBlock) {
+      for (int slot = tid; slot < 4 * W; slot += BLOCK) {
         const int w = slot >> 2, q = slot & 3;
         unsigned bits = (unsigned)((cur[w] >> (16 * q)) & 0xFFFFull);
         while (bits) {
@@ -688,7 +694,7 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
         }
       }
     }
-    for (int w = tid; w < W; w += kLdsBlock) clr[w] = 0;
+    for (int w = tid; w < W; w += BLOCK) clr[w] = 0;
     if (tid == 0) s_qn[(r + 1) % 3] = 0;
     __syncthreads();
     if (!s_qn[r % 3]) break;
@@ -707,14 +713,14 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
   //      costs one L2 round trip.  Results stay in registers until every
   //      thread is done reading offsets (pred shares their space). ----
   __shared__ int s_level_ties;
-  for (int v = tid; v <= N; v += kLdsBlock) offs[v] = (unsigned short)A.rv_off[v];
-  for (int w = tid; w < W; w += kLdsBlock) bmB[w] = 0;     // nodes left for attach_level_ties
+  for (int v = tid; v <= N; v += BLOCK) offs[v] = (unsigned short)A.rv_off[v];
+  for (int w = tid; w < W; w += BLOCK) bmB[w] = 0;     // nodes left for attach_level_ties
   if (tid == 0) s_level_ties = 0;
   __syncthreads();
   unsigned short my_tail[kLdsPerThread], my_pred[kLdsPerThread];
   {
     int j = 0;
-    for (int v = tid; v < N; v += kLdsBlock, ++j) {
+    for (int v = tid; v < N; v += BLOCK, ++j) {
       unsigned short p = kNoPred, pt = 0;
       const unsigned long long dv = dist[v];
       if (v != src && dv != kMax) {
@@ -766,11 +772,11 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
   const long long clk2 = wall_clock64();
   {
     int j = 0;
-    for (int v = tid; v < N; v += kLdsBlock, ++j) { pred[v] = my_pred[j]; ptail[v] = my_tail[j]; }
+    for (int v = tid; v < N; v += BLOCK, ++j) { pred[v] = my_pred[j]; ptail[v] = my_tail[j]; }
   }
   if (s_level_ties) {
     __shared__ int s_left, s_added;
-    const bool ok = attach_level_ties<kLdsBlock>(
+    const bool ok = attach_level_ties<BLOCK>(
         A, rw, src, bmB, bmC, &s_left, &s_added, [&](int v) { return dist[v]; },
         [&](int v, int e, int u) {
           pred[v] = (unsigned short)e;
@@ -800,14 +806,14 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
   //      Integer demands make every partial sum exact, so the order of the
   //      LDS atomics does not matter. ----
   double* flow = (double*)dist;
-  for (int v = tid; v < N; v += kLdsBlock) flow[v] = 0.0;
+  for (int v = tid; v < N; v += BLOCK) flow[v] = 0.0;
   __shared__ unsigned long long s_steps;
   __shared__ int s_maxlen, s_loaded;
   if (tid == 0) { s_steps = 0; s_maxlen = 0; s_loaded = 0; }
   __syncthreads();
   unsigned long long steps = 0;
   int maxlen = 0;
-  for (long long i = A.od_start[src] + tid; i < A.od_start[src + 1]; i += kLdsBlock) {
+  for (long long i = A.od_start[src] + tid; i < A.od_start[src + 1]; i += BLOCK) {
     const int d = A.od_dest[i];
     const double f = A.od_val[i];
     if (d == src || !(f > 0.0)) continue;
@@ -831,7 +837,7 @@ __global__ __launch_bounds__(kLdsBlock) void k_origin_lds(OriginArgs A, int load
   __syncthreads();
   double* __restrict__ yv = A.y + (size_t)var * A.E;
   int loaded = 0;
-  for (int v = tid; v < N; v += kLdsBlock) {
+  for (int v = tid; v < N; v += BLOCK) {
     const double f = flow[v];
     if (f != 0.0 && pred[v] != kNoPred) {
       unsafeAtomicAdd(&yv[pred[v]], f);
@@ -1239,6 +1245,15 @@ struct VarState {
   int iter, active, stepping, need_more, jnext, found;
 };
 
+// Arguments of a tapgpu_fw_run call that the decision kernel reads from
+// device memory, so a captured iteration graph serves every call.
+struct FwParams {
+  tapgpu_iter* trace;
+  int trace_cap;
+  int max_iters;
+  double target;
+};
+
 // mode 0: c = t0 (free-flow, initial Wa); mode 1: c = BPR(x); mode 2: c given
 __global__ __launch_bounds__(kBlock) void k_costs(Links K, int mode, const unsigned char* var_on,
                                                   int v0, int Vn) {
@@ -1328,8 +1343,10 @@ __global__ __launch_bounds__(kBlock) void k_link_partials(Links K, const VarStat
 // excess cost, trace record, stop test.
 __global__ __launch_bounds__(kBlock) void k_link_decide(VarState* st, const tap_acc* part,
                                                         int nparts, int only_g,
-                                                        tapgpu_iter* trace, int trace_cap,
-                                                        int max_iters, double target) {
+                                                        const FwParams* prm) {
+  tapgpu_iter* const trace = prm->trace;
+  const int trace_cap = prm->trace_cap, max_iters = prm->max_iters;
+  const double target = prm->target;
   const int var = blockIdx.x;
   VarState s = st[var];
   if (!s.stepping || (only_g && !s.need_more)) return;
@@ -1415,6 +1432,31 @@ __global__ __launch_bounds__(kBlock) void k_link_step(Links K, VarState* st) {
   }
 }
 
+// Total link cost sum_k x_k c_k of every variant (one block each): the
+// compensated sums of the threads' strided links, merged in a fixed order,
+// so the result does not depend on timing.  Absent links carry x = 0.
+// Per variant at the current volumes: out[2v] = VHT = sum x * c, out[2v+1]
+// = the Beckmann objective (sum of the link cost integrals), compensated
+// sums in a fixed order.
+__global__ __launch_bounds__(kBlock) void k_totals(Links K, double* out) {
+  const int var = blockIdx.x;
+  tap_acc acc[kQ];
+  for (int q = 0; q < kQ; ++q) acc[q] = tap_acc{0.0, 0.0};
+  const size_t off = (size_t)var * K.E;
+  for (int k = threadIdx.x; k < K.E; k += kBlock) {
+    const size_t i = off + k;
+    if (!K.present[i]) continue;
+    tap_acc_add(&acc[0], K.x[i] * K.c[i]);
+    tap_acc_add(&acc[1], tap_integral(K.t0[i], K.cap[i], K.B[i], K.pw[i], K.x[i]));
+  }
+  __shared__ tap_acc sh[kBlock / kWave * kQ];
+  block_reduce_acc(acc, sh);
+  if (threadIdx.x == 0) {
+    out[2 * var] = tap_acc_value(acc[0]);
+    out[2 * var + 1] = tap_acc_value(acc[1]);
+  }
+}
+
 __global__ void k_finish_step(VarState* st, int V, unsigned char* var_on) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= V) return;
@@ -1479,7 +1521,9 @@ struct tapgpu_ctx {
   double* od_val = nullptr;
   // strategy: 0 auto, 1 node-major sweeps, 2 one origin per workgroup (LDS)
   int strategy = 0;
-  int chase = 0;                      // LDS path: chained expansions (-1: tune)
+  int chase = -1;                     // LDS path: chained expansions (-1: tune)
+  int lds_block = -1;                 // LDS path: workgroup size (-1: tune)
+  bool qcap_auto = true;              // LDS path: queue size picked with the workgroup size
   int split = 0;                      // LDS path: origins per variant run by the HBM kernel
                                       // alongside it (-1: tune, 0: none)
   hipStream_t stream2 = nullptr;      // second queue for that co-running kernel
@@ -1520,6 +1564,15 @@ struct tapgpu_ctx {
   unsigned char* var_on = nullptr;
   VarState* st = nullptr;
   tap_acc* part = nullptr;
+  double* vht = nullptr;              // [2V] VHT, objective at the current volumes
+  FwParams* prm = nullptr;            // the fw_run call's limits and trace
+  // captured iteration graphs (replayed while profiling is off):
+  // g_iter = iteration begin + per-origin loading + link passes + readback,
+  // g_link = link passes + readback (for the lanes path)
+  bool graphs = true;
+  hipGraphExec_t g_iter = nullptr, g_link = nullptr;
+  long long g_iter_key[6] = {-1, -1, -1, -1, -1, -1};
+  int64_t graph_launches = 0;
   tapgpu_iter* trace = nullptr;
   int trace_cap = 0;
   bool initialised = false;
@@ -1549,6 +1602,7 @@ struct tapgpu_ctx {
   bool lanes_ok = false;              // its LDS (bitmaps + queues) fits
   int lanes_qcap = 0;                 // LDS frontier queue entries per round
   int lanes_qcap_max = 0;             // entries its LDS allocation holds
+  double lanes_delta_mult = -1;       // bucket width / mean free-flow time (-1: tune)
   size_t lanes_lds = 0;
   int lanes_grid = 0;                 // persistent workgroups (rows allocated)
   int lanes_grid_max = 0;             // resident workgroups on this device
@@ -1556,7 +1610,18 @@ struct tapgpu_ctx {
   double *l_W = nullptr, *l_Y = nullptr;                     // [groups][E][64]
   int* l_lane_var = nullptr;          // [groups * 64]
   std::vector<int> h_lane_var;        // its host copy (as uploaded)
+  // multi-origin kernel (strategy 4)
+  bool multi_ok = false;              // its LDS (bitmaps + queues) fits
+  int multi_qcap = 0, multi_qcap_max = 0;
+  size_t multi_lds = 0;
+  int multi_grid = 0, multi_grid_max = 0;
+  int multi_k = -1;                   // origins per task (-1: tune)
+  double multi_delta_mult = -1;       // bucket width / mean free-flow time (-1: tune)
+  unsigned long long* m_dist = nullptr;
+  int *m_pred = nullptr, *m_ptail = nullptr, *m_zones = nullptr;
   unsigned l_stamp = 1;               // stamp of the next task's memo entries
+  unsigned* l_stamp_dev = nullptr;    // its device copy, read by the kernel
+  bool capturing = false;             // launches are being recorded into a graph
   // stats
   tapgpu_stats stats{};
   bool profiling = false;
@@ -1800,8 +1865,12 @@ bool use_lds(tapgpu_ctx* c) {
 
 // auto: labels in LDS where they fit, else labels in HBM
 bool use_hbm(tapgpu_ctx* c) {
-  return c->hbm_ok && (c->strategy == 3 || (c->strategy == 0 && !c->lds_ok));
+  return c->hbm_ok && (c->strategy == 3 || c->strategy == 4 || (c->strategy == 0 && !c->lds_ok));
 }
+
+// strategy 4: loadings by the multi-origin kernel (shortest-path-only calls
+// and its level-tie fallback use the labels-in-HBM kernel)
+bool use_multi(tapgpu_ctx* c) { return c->multi_ok && c->strategy == 4; }
 
 OriginArgs make_origin_args(tapgpu_ctx* c, const unsigned char* var_on) {
   OriginArgs A{};
@@ -1818,6 +1887,21 @@ OriginArgs make_origin_args(tapgpu_ctx* c, const unsigned char* var_on) {
   A.z0 = 0;
   A.zn = c->Z;
   return A;
+}
+
+// Workgroup size the LDS kernel runs with (before tuning: kLdsBlock).
+int lds_threads(const tapgpu_ctx* c) { return c->lds_block > 0 ? c->lds_block : kLdsBlock; }
+
+// One launch of the LDS kernel at the context's workgroup size, with the
+// LDS its queue size needs (a smaller queue leaves room for more
+// workgroups per CU).
+void launch_lds(tapgpu_ctx* c, dim3 grid, hipStream_t st, const OriginArgs& A, int load) {
+  const size_t sh = origin_lds_bytes(c->N, A.qcap);
+  switch (lds_threads(c)) {
+    case 256: hipLaunchKernelGGL(k_origin_lds<256>, grid, dim3(256), sh, st, A, load); break;
+    case 512: hipLaunchKernelGGL(k_origin_lds<512>, grid, dim3(512), sh, st, A, load); break;
+    default: hipLaunchKernelGGL(k_origin_lds<1024>, grid, dim3(1024), sh, st, A, load); break;
+  }
 }
 
 // Shortest paths + loading for variants [v0, v0+Vn) with one workgroup per
@@ -1864,7 +1948,7 @@ int launch_origin(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on, fl
       HIPCHK(c, hipGetLastError());
       HIPCHK(c, hipEventRecord(c->ev_join, c->stream2));
     }
-    hipLaunchKernelGGL(k_origin_lds, dim3(zl, Vn), dim3(kLdsBlock), c->lds_bytes, c->stream, A, 1);
+    launch_lds(c, dim3(zl, Vn), c->stream, A, 1);
     HIPCHK(c, hipGetLastError());
     if (zh > 0) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     if (ms) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
@@ -1926,6 +2010,47 @@ int tune_chase(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
   return TAPGPU_OK;
 }
 
+// Workgroup size and queue size of the LDS kernel: every allowed size
+// (kLdsPerThread nodes per thread at most) with, unless the queue size was
+// set, the full queue and two shorter ones (less LDS per workgroup, so more
+// workgroups per CU), each timed on one full launch; the fastest pair is
+// kept.  Results do not depend on them.
+int tune_lds_shape(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
+  if (!c->ev[0])
+    for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
+  std::vector<int> qs{c->qcap};
+  if (c->qcap_auto) {
+    qs = {c->qcap_max};
+    for (int q : {1024, 256})
+      if (q < c->qcap_max) qs.push_back(q);
+  }
+  float best_ms = 0.f;
+  int best_b = kLdsBlock, best_q = c->qcap;
+  for (int b : kLdsBlocks) {
+    if (c->N > kLdsPerThread * b) continue;
+    for (int q : qs) {
+      c->lds_block = b;
+      c->qcap = q;
+      float ms = 0.f;
+      int rc = launch_origin(c, v0, Vn, var_on, &ms);
+      if (rc) return rc;
+      int herr = 0;
+      HIPCHK(c, hipMemcpyAsync(&herr, c->err_dev, sizeof herr, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      if (herr) {                       // reported by the caller's launch
+        c->lds_block = best_b;
+        c->qcap = best_q;
+        return TAPGPU_OK;
+      }
+      if (best_ms == 0.f || ms < best_ms) { best_ms = ms; best_b = b; best_q = q; }
+    }
+  }
+  c->lds_block = best_b;
+  c->qcap = best_q;
+  c->stats.origin_tune_ms += best_ms;
+  return TAPGPU_OK;
+}
+
 int absorb_origin(tapgpu_ctx* c, const int* herr, const unsigned long long* hc, int Vn,
                   std::chrono::steady_clock::time_point t0);
 
@@ -1976,6 +2101,8 @@ int absorb_origin(tapgpu_ctx* c, const int* herr, const unsigned long long* hc, 
   }
   c->stats.origin_blocks += (int64_t)c->Z * Vn;
   c->stats.origin_chase = c->chase;
+  c->stats.origin_block = use_lds(c) ? lds_threads(c) : 0;
+  c->stats.origin_queue = use_lds(c) ? c->qcap : c->hqcap;
   c->stats.origin_split = c->split;
   c->stats.aon_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (herr[0] == 1)
@@ -1991,9 +2118,19 @@ int absorb_origin(tapgpu_ctx* c, const int* herr, const unsigned long long* hc, 
 int aon_lds(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
   auto t0 = std::chrono::steady_clock::now();
   if (!c->pre_cleared) HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
+  if (c->lds_block < 0 && c->Z > 0) {
+    // workgroup and queue size first, then the chain length, on the LDS
+    // kernel alone
+    const int keep = c->split;
+    c->split = 0;
+    int rc = tune_lds_shape(c, v0, Vn, var_on);
+    c->split = keep;
+    if (rc) return rc;
+    HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
+  }
   if (c->chase < 0 && c->Z > 0) {
     const int keep = c->split;
-    c->split = 0;                       // chain length first, on the LDS kernel alone
+    c->split = 0;
     int rc = tune_chase(c, v0, Vn, var_on);
     c->split = keep;
     if (rc) return rc;
@@ -2095,6 +2232,7 @@ int aon_hbm(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
 }
 
 #include "taplanes.inc"
+#include "tapmulti.inc"
 
 // The lanes kernel loads a batch when at least lanes_min of its variants
 // are active (lanes_min 0: never).
@@ -2108,30 +2246,39 @@ bool use_lanes(tapgpu_ctx* c, int v0, int Vn, const unsigned char* h_on) {
 // All-or-nothing loading of variants [v0, v0 + Vn) (groups of those
 // variants) with the current costs, by the configured strategy.  h_on is
 // the host copy of var_on (null: every variant of the range).
-int load_variants(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on,
-                  const unsigned char* h_on) {
-  if (use_lanes(c, v0, Vn, h_on)) {
-    bool fallback = false;
-    int rc = aon_lanes(c, v0, Vn, var_on, h_on, &fallback);
-    if (rc || !fallback) return rc;
-    // a level tie: the per-origin kernels redo the loading from cleared
-    // volumes, error flag and counters
-    const bool pc = c->pre_cleared;
-    c->pre_cleared = false;
-    if (use_lds(c)) rc = aon_lds(c, v0, Vn, var_on);
-    else if (use_hbm(c)) rc = aon_hbm(c, v0, Vn, var_on);
-    else {
-      rc = sssp_groups(c, v0 * c->Gv, Vn * c->Gv, c->col_src, c->grp_var, var_on);
-      if (!rc) rc = aon_groups(c, v0 * c->Gv, Vn * c->Gv, v0, Vn, var_on);
-    }
-    c->pre_cleared = pc;
-    return rc;
-  }
+int load_per_origin(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
   if (use_lds(c)) return aon_lds(c, v0, Vn, var_on);
   if (use_hbm(c)) return aon_hbm(c, v0, Vn, var_on);
   int rc = sssp_groups(c, v0 * c->Gv, Vn * c->Gv, c->col_src, c->grp_var, var_on);
   if (!rc) rc = aon_groups(c, v0 * c->Gv, Vn * c->Gv, v0, Vn, var_on);
   return rc;
+}
+
+// A loading redone by the per-origin strategy after the lanes kernel met a
+// level tie: from cleared volumes, error flag and counters.
+int redo_per_origin(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
+  const bool pc = c->pre_cleared;
+  c->pre_cleared = false;
+  const int rc = load_per_origin(c, v0, Vn, var_on);
+  c->pre_cleared = pc;
+  return rc;
+}
+
+int load_variants(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on,
+                  const unsigned char* h_on) {
+  if (use_multi(c) && !use_lanes(c, v0, Vn, h_on)) {
+    bool fallback = false;
+    int rc = aon_multi(c, v0, Vn, var_on, &fallback);
+    if (rc || !fallback) return rc;
+    return redo_per_origin(c, v0, Vn, var_on);      // (strategy 4 redoes with k_origin_hbm)
+  }
+  if (use_lanes(c, v0, Vn, h_on)) {
+    bool fallback = false;
+    int rc = aon_lanes(c, v0, Vn, var_on, h_on, &fallback);
+    if (rc || !fallback) return rc;
+    return redo_per_origin(c, v0, Vn, var_on);
+  }
+  return load_per_origin(c, v0, Vn, var_on);
 }
 
 int launch_costs(tapgpu_ctx* c, int mode, const unsigned char* var_on, int v0, int Vn) {
@@ -2353,7 +2500,7 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
       if (c->lanes_ok) {
         const size_t share = std::min<size_t>((size_t)lds_cu / kLanesPerCu, (size_t)lds_max) - 256;
         const size_t spare = share > base ? share - base : 0;
-        c->lanes_qcap_max = (int)std::min<size_t>(kLanesQueueMax, spare / 8) & ~63;
+        c->lanes_qcap_max = (int)std::min<size_t>(kLanesQueueMax, spare / 24) & ~63;
         c->lanes_qcap = c->lanes_qcap_max;
         c->lanes_lds = lanes_lds_bytes(c->N, c->lanes_qcap_max);
         int pc = 0;
@@ -2367,6 +2514,34 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
         c->lanes_grid_max = std::max(1, ncu * pc);
       }
       c->lanes_min = V > 1 ? kLanesMinDefault : 0;
+    }
+    // multi-origin kernel: three bitmaps, then queues from the LDS share of
+    // the kMultiPerCu workgroups on a CU
+    {
+      int lds_cu = 0;
+      if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
+              hipSuccess || lds_cu <= 0)
+        lds_cu = lds_max;
+      const size_t base = multi_lds_bytes(c->N, 0);
+      const size_t share = std::min<size_t>((size_t)lds_cu / kMultiPerCu, (size_t)lds_max) - 256;
+      c->multi_ok = base + 64 * 24 <= share;
+      if (c->multi_ok) {
+        c->multi_qcap_max = (int)std::min<size_t>(kMultiQueueMax, (share - base) / 24) & ~63;
+        c->multi_qcap = c->multi_qcap_max;
+        c->multi_lds = multi_lds_bytes(c->N, c->multi_qcap_max);
+        int pc = 0;
+        const void* fns[] = {reinterpret_cast<const void*>(k_origin_multi<4>),
+                             reinterpret_cast<const void*>(k_origin_multi<8>),
+                             reinterpret_cast<const void*>(k_origin_multi<16>)};
+        bool ok = true;
+        for (const void* f : fns)
+          ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)c->multi_lds) == hipSuccess;
+        if (!ok || hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k_origin_multi<8>, kMultiBlock,
+                                                                 c->multi_lds) != hipSuccess || pc < 1)
+          pc = 1;
+        c->multi_grid_max = std::max(1, ncu * pc);
+      }
     }
     double sum = 0;
     int cnt = 0;
@@ -2440,6 +2615,7 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
 
 
   ALLOC(part, (size_t)V * kLinkBlocks * kQ);
+  ALLOC(prm, 1);
 #undef ALLOC
   if ((rc = upload(c, c->fw_off, fw_off.data(), N + 1))) return rc;
   if ((rc = upload(c, c->fw_dst, fw_dst.data(), E))) return rc;
@@ -2462,9 +2638,14 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
   if ((rc = upload(c, c->od_start, h_od_start.data(), Z + 1))) return rc;
   if ((rc = upload(c, c->od_dest, h_od_dest.data(), h_od_dest.size()))) return rc;
   if ((rc = upload(c, c->od_val, h_od_val.data(), h_od_val.size()))) return rc;
-  if (c->lds_ok)
-    HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_origin_lds),
+  if (c->lds_ok) {
+    HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_origin_lds<1024>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_bytes));
+    HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_origin_lds<512>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_bytes));
+    HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_origin_lds<256>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_bytes));
+  }
 
   c->h_od_start = std::move(h_od_start);
   c->h_od_dest = std::move(h_od_dest);
@@ -2520,6 +2701,8 @@ int tapgpu_create(tapgpu_ctx** out, const tapgpu_network* net, const tapgpu_dema
 void tapgpu_destroy(tapgpu_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->g_iter) (void)hipGraphExecDestroy(c->g_iter);
+  if (c->g_link) (void)hipGraphExecDestroy(c->g_link);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->trace) (void)hipFree(c->trace);
   if (c->h_int) (void)hipHostFree(c->h_int);
@@ -2602,8 +2785,7 @@ int tapgpu_sssp(tapgpu_ctx* c, int64_t variant, const int64_t* start_nodes, int6
         hipLaunchKernelGGL(k_origin_hbm, dim3((unsigned)std::min<int64_t>(ns, c->hbm_grid)),
                            dim3(kHbmBlock), c->hbm_lds, c->stream, A, 0, (int)ns);
       else
-        hipLaunchKernelGGL(k_origin_lds, dim3((unsigned)ns, 1), dim3(kLdsBlock), c->lds_bytes,
-                           c->stream, A, 0);
+        launch_lds(c, dim3((unsigned)ns, 1), c->stream, A, 0);
       int herr[3] = {0, 0, 0};
       const hipError_t e0 = hipGetLastError();
       const hipError_t e1 = hipMemcpyAsync(hd.data(), d_dist, sizeof(double) * c->N * chunk,
@@ -2808,8 +2990,7 @@ int tapgpu_sssp_noalloc(tapgpu_ctx* c, int64_t variant, const int64_t* d_start_n
   hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(npred / kWave + 1)), dim3(kBlock), 0, c->stream,
                      d_pred, npred, -1);
   if (lds)
-    hipLaunchKernelGGL(k_origin_lds, dim3((unsigned)num_start, 1), dim3(kLdsBlock), c->lds_bytes,
-                       c->stream, A, 0);
+    launch_lds(c, dim3((unsigned)num_start, 1), c->stream, A, 0);
   else
     hipLaunchKernelGGL(k_origin_hbm, dim3((unsigned)std::min<int64_t>(num_start, rows)),
                        dim3(kHbmBlock), c->hbm_lds, c->stream, A, 0, (int)num_start);
@@ -2899,6 +3080,138 @@ int tapgpu_fw_init(tapgpu_ctx* c, const double* warm_volumes) {
   return TAPGPU_OK;
 }
 
+__global__ void k_set_params(FwParams* p, FwParams v) { *p = v; }
+
+// (a kernel argument, not a copy from host memory that the next call
+// overwrites)
+static int upload_params(tapgpu_ctx* c, int max_iters, double target) {
+  hipLaunchKernelGGL(k_set_params, dim3(1), dim3(1), 0, c->stream, c->prm,
+                     FwParams{c->trace, c->trace_cap, max_iters, target});
+  HIPCHK(c, hipGetLastError());
+  return TAPGPU_OK;
+}
+
+// The link passes of an iteration: two line-search windows (the second does
+// nothing for variants the first decided), the step, the flags, and the
+// copy of the iteration block (error flag, counters, states, flags) to
+// pinned host memory.
+static int issue_link_passes(tapgpu_ctx* c, const Links& K, dim3 lgrid) {
+  hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kBlock), 0, c->stream, K, c->st, c->part, 0);
+  hipLaunchKernelGGL(k_link_decide, dim3(c->V), dim3(kBlock), 0, c->stream, c->st, c->part,
+                     kLinkBlocks, 0, c->prm);
+  hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kBlock), 0, c->stream, K, c->st, c->part, 1);
+  hipLaunchKernelGGL(k_link_decide, dim3(c->V), dim3(kBlock), 0, c->stream, c->st, c->part,
+                     kLinkBlocks, 1, c->prm);
+  hipLaunchKernelGGL(k_link_step, lgrid, dim3(kBlock), 0, c->stream, K, c->st);
+  hipLaunchKernelGGL(k_finish_step, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->st,
+                     c->V, c->var_on);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(c->pin_block, c->iter_dev, c->iter_bytes, hipMemcpyDeviceToHost,
+                           c->stream));
+  return TAPGPU_OK;
+}
+
+// Records what `issue` puts on the context's stream into an executable
+// graph (replacing *exec).  Nothing in a captured sequence may allocate,
+// synchronise, record events or clear memory with a memset: the captured
+// kernels clear what they need themselves.
+extern "C++" template <class F>
+static int capture(tapgpu_ctx* c, hipGraphExec_t* exec, F&& issue) {
+  if (*exec) {
+    HIPCHK(c, hipGraphExecDestroy(*exec));
+    *exec = nullptr;
+  }
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = issue();
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess || !g) return c->fail(TAPGPU_E_HIP, "stream capture failed: %s", hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    *exec = nullptr;
+    return c->fail(TAPGPU_E_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+  }
+  return TAPGPU_OK;
+}
+
+// The whole iteration of a tuned per-origin schedule as one graph:
+// k_iter_begin, the loading kernel, the link passes and the readback.  It is
+// captured again when the schedule that its launches bake in changes.
+static int iter_graph(tapgpu_ctx* c, const Links& K, dim3 lgrid) {
+  const bool lds = use_lds(c);
+  if (!lds) {
+    int rc = ensure_hbm(c);
+    if (rc) return rc;
+  }
+  long long dbits;
+  std::memcpy(&dbits, &c->delta_mult, sizeof dbits);
+  const long long key[6] = {lds ? 1 : 2, c->chase, c->qcap, c->hqcap, dbits, lds_threads(c)};
+  if (c->g_iter && std::equal(key, key + 6, c->g_iter_key)) return TAPGPU_OK;
+  int rc = capture(c, &c->g_iter, [&]() {
+    const long long ny = (long long)c->V * c->E;
+    hipLaunchKernelGGL(k_iter_begin, dim3(grid_for(ny / kWave + 1)), dim3(kBlock), 0, c->stream,
+                       c->st, c->V, c->var_on, c->err_dev, c->counters, c->y, ny);
+    c->defer_origin = true;
+    c->pre_cleared = true;
+    int r = lds ? aon_lds(c, 0, c->V, c->var_on) : aon_hbm(c, 0, c->V, c->var_on);
+    c->defer_origin = false;
+    c->pre_cleared = false;
+    c->origin_pending = false;
+    if (r) return r;
+    return issue_link_passes(c, K, lgrid);
+  });
+  if (rc) return rc;
+  std::copy(key, key + 6, c->g_iter_key);
+  return TAPGPU_OK;
+}
+
+// The whole iteration of a batch on the lanes kernel as one graph:
+// k_iter_begin (flags, error flag, counters), pack, the lanes kernel, unpack,
+// the gate, the link passes and the readback.  The lane map and the memo
+// stamps are device data, set before each replay; the graph is captured
+// again when the number of groups or the kernel's schedule changes.
+static int lanes_graph(tapgpu_ctx* c, const Links& K, dim3 lgrid, int G) {
+  long long dbits;
+  std::memcpy(&dbits, &c->lanes_delta_mult, sizeof dbits);
+  const long long key[6] = {3, G, c->lanes_qcap, dbits, (long long)c->lanes_lds, c->lanes_grid};
+  if (c->g_iter && std::equal(key, key + 6, c->g_iter_key)) return TAPGPU_OK;
+  c->capturing = true;
+  int rc = capture(c, &c->g_iter, [&]() {
+    hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(kBlock), 0, c->stream, c->st, c->V, c->var_on,
+                       c->err_dev, c->counters, c->y, 0ll);
+    int r = launch_lanes(c, G, (long long)c->Z * G, nullptr);
+    if (r) return r;
+    hipLaunchKernelGGL(k_lanes_gate, dim3(1), dim3(kBlock), 0, c->stream, c->err_dev, c->st, c->V);
+    return issue_link_passes(c, K, lgrid);
+  });
+  c->capturing = false;
+  if (rc) return rc;
+  std::copy(key, key + 6, c->g_iter_key);
+  return TAPGPU_OK;
+}
+
+static int link_graph(tapgpu_ctx* c, const Links& K, dim3 lgrid) {
+  if (c->g_link) return TAPGPU_OK;
+  return capture(c, &c->g_link, [&]() { return issue_link_passes(c, K, lgrid); });
+}
+
+int tapgpu_set_graphs(tapgpu_ctx* c, int on) {
+  if (!c) return TAPGPU_E_ARG;
+  c->graphs = on != 0;
+  return TAPGPU_OK;
+}
+
+int tapgpu_get_graphs(tapgpu_ctx* c, int* on) {
+  if (!c || !on) return TAPGPU_E_ARG;
+  *on = c->graphs ? 1 : 0;
+  return TAPGPU_OK;
+}
+
 int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_result* results) {
   if (!c) return TAPGPU_E_ARG;
   if (!c->initialised) return c->fail(TAPGPU_E_STATE, "tapgpu_fw_init was not called");
@@ -2927,6 +3240,7 @@ int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_res
   // below (rare: lambda below 2^-15).
   if (!c->ev[2])
     for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
+  if ((rc = upload_params(c, max_iters, target_relgap))) return rc;
   for (;;) {
     bool any = false;
     for (int v = 0; v < c->V; ++v) any = any || hon[v];
@@ -2934,40 +3248,85 @@ int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_res
     if (c->trace_cap < 1 << 30) {
       int maxit = 0;
       for (int v = 0; v < c->V; ++v) maxit = std::max(maxit, hs[v].iter);
-      if (maxit + 1 >= c->trace_cap && (rc = ensure_trace(c, 2 * c->trace_cap))) return rc;
+      if (maxit + 1 >= c->trace_cap) {
+        if ((rc = ensure_trace(c, 2 * c->trace_cap))) return rc;
+        if ((rc = upload_params(c, max_iters, target_relgap))) return rc;
+      }
     }
     auto t_it = std::chrono::steady_clock::now();
     // with the schedule already tuned, one kernel opens the iteration and
     // clears what the loading would clear
-    const bool tuned = use_lds(c) ? (c->chase >= 0 && c->split >= 0)
-                                  : (use_hbm(c) && c->delta_mult >= 0);
-    if (tuned) {
-      const long long ny = (long long)c->V * c->E;
-      hipLaunchKernelGGL(k_iter_begin, dim3(grid_for(ny / kWave + 1)), dim3(kBlock), 0, c->stream, c->st,
-                         c->V, c->var_on, c->err_dev, c->counters, c->y, ny);
-    } else {
-      hipLaunchKernelGGL(k_begin_step, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->st, c->V,
-                         c->var_on);
+    const bool multi_now = use_multi(c);
+    const bool tuned = multi_now ? (c->multi_k > 0 && c->multi_delta_mult >= 0)
+                       : use_lds(c) ? (c->chase >= 0 && c->split >= 0 && c->lds_block > 0)
+                                    : (use_hbm(c) && c->delta_mult >= 0);
+    const bool graphs = c->graphs && !c->profiling;
+    const bool lanes_now = use_lanes(c, 0, c->V, hon.data());
+    float aon = 0.f;
+    // how the iteration runs: a captured graph of all of it (per-origin
+    // kernel or lanes kernel), or the loading issued launch by launch and
+    // the link passes (captured or not) after it
+    int mode = 0;
+    int G = 0;
+    if (graphs && lanes_now && c->lanes_delta_mult >= 0 && c->l_dist && c->Z > 0) {
+      if ((rc = lanes_map(c, 0, c->V, hon.data(), &G))) return rc;
+      if (G > 0) mode = 2;
+    } else if (graphs && tuned && !lanes_now && !multi_now &&
+               (use_lds(c) ? c->split == 0 : use_hbm(c))) {
+      mode = 1;
     }
-    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-    c->defer_origin = true;
-    c->pre_cleared = tuned;
-    rc = load_variants(c, 0, c->V, c->var_on, hon.data());
-    c->defer_origin = false;
-    c->pre_cleared = false;
-    if (rc) return rc;
-    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-    hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kBlock), 0, c->stream, K, c->st, c->part, 0);
-    hipLaunchKernelGGL(k_link_decide, dim3(c->V), dim3(kBlock), 0, c->stream, c->st, c->part,
-                       kLinkBlocks, 0, c->trace, c->trace_cap, max_iters, target_relgap);
-    hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kBlock), 0, c->stream, K, c->st, c->part, 1);
-    hipLaunchKernelGGL(k_link_decide, dim3(c->V), dim3(kBlock), 0, c->stream, c->st, c->part,
-                       kLinkBlocks, 1, c->trace, c->trace_cap, max_iters, target_relgap);
-    hipLaunchKernelGGL(k_link_step, lgrid, dim3(kBlock), 0, c->stream, K, c->st);
-    hipLaunchKernelGGL(k_finish_step, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->st,
-                       c->V, c->var_on);
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(c->pin_block, c->iter_dev, c->iter_bytes, hipMemcpyDeviceToHost, c->stream));
+    if (mode == 1) {
+      // the loading's error flag and counters come back in the iteration block
+      if ((rc = iter_graph(c, K, lgrid))) return rc;
+      HIPCHK(c, hipGraphLaunch(c->g_iter, c->stream));
+      c->graph_launches += 1;
+      c->origin_pending = true;
+      c->pending_Vn = c->V;
+      if (!use_lds(c)) c->stats.origin_delta = c->delta_mult;
+    } else if (mode == 2) {
+      const long long tasks = (long long)c->Z * G;
+      if ((rc = lanes_reserve(c, tasks))) return rc;
+      if ((rc = lanes_graph(c, K, lgrid, G))) return rc;
+      HIPCHK(c, hipGraphLaunch(c->g_iter, c->stream));
+      c->graph_launches += 1;
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      bool fallback = false;
+      if ((rc = absorb_lanes(c, c->pin_err, c->pin_cnt, tasks, c->V, std::chrono::steady_clock::now(),
+                             &fallback)))
+        return rc;
+      if (fallback) {
+        // the gate kept every variant from stepping: open the step again,
+        // load with a per-origin kernel and run the link passes
+        hipLaunchKernelGGL(k_begin_step, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->st,
+                           c->V, c->var_on);
+        if ((rc = redo_per_origin(c, 0, c->V, c->var_on))) return rc;
+        if ((rc = issue_link_passes(c, K, lgrid))) return rc;
+      }
+    } else {
+      if (tuned) {
+        const long long ny = (long long)c->V * c->E;
+        hipLaunchKernelGGL(k_iter_begin, dim3(grid_for(ny / kWave + 1)), dim3(kBlock), 0, c->stream,
+                           c->st, c->V, c->var_on, c->err_dev, c->counters, c->y, ny);
+      } else {
+        hipLaunchKernelGGL(k_begin_step, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->st,
+                           c->V, c->var_on);
+      }
+      HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+      c->defer_origin = true;
+      c->pre_cleared = tuned;
+      rc = load_variants(c, 0, c->V, c->var_on, hon.data());
+      c->defer_origin = false;
+      c->pre_cleared = false;
+      if (rc) return rc;
+      HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+      if (graphs) {
+        if ((rc = link_graph(c, K, lgrid))) return rc;
+        HIPCHK(c, hipGraphLaunch(c->g_link, c->stream));
+        c->graph_launches += 1;
+      } else if ((rc = issue_link_passes(c, K, lgrid))) {
+        return rc;
+      }
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if ((rc = absorb_pending(c))) return rc;
     c->stats.link_passes += 1;
@@ -2981,7 +3340,7 @@ int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_res
     while (more) {
       hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kBlock), 0, c->stream, K, c->st, c->part, 1);
       hipLaunchKernelGGL(k_link_decide, dim3(c->V), dim3(kBlock), 0, c->stream, c->st, c->part,
-                         kLinkBlocks, 1, c->trace, c->trace_cap, max_iters, target_relgap);
+                         kLinkBlocks, 1, c->prm);
       hipLaunchKernelGGL(k_link_step, lgrid, dim3(kBlock), 0, c->stream, K, c->st);
       hipLaunchKernelGGL(k_finish_step, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->st,
                          c->V, c->var_on);
@@ -2995,31 +3354,38 @@ int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_res
     }
     std::copy(c->pin_st, c->pin_st + c->V, hs.begin());
     std::copy(c->pin_on, c->pin_on + c->V, hon.begin());
-    float aon = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&aon, c->ev[2], c->ev[3]));
     const double it_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_it).count();
+    // (a captured iteration is timed as a whole, in link_ms)
+    if (mode == 0) HIPCHK(c, hipEventElapsedTime(&aon, c->ev[2], c->ev[3]));
     c->stats.aon_ms += aon;
     c->stats.link_ms += std::max(0.0, it_ms - aon);
   }
+  c->stats.graph_launches = c->graph_launches;
+  c->graph_launches = 0;
   c->stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   if (results) {
-    HIPCHK(c, hipMemcpy(hs.data(), c->st, sizeof(VarState) * c->V, hipMemcpyDeviceToHost));
-    std::vector<double> x((size_t)c->E), cc((size_t)c->E);
+    // VHT after the last step: sum x*c per variant on the device (fixed
+    // order), read back with the variant states
+    if (!c->vht) {
+      int rc2 = dalloc(c, &c->vht, 2 * (size_t)c->V);
+      if (rc2) return rc2;
+    }
+    hipLaunchKernelGGL(k_totals, dim3(c->V), dim3(kBlock), 0, c->stream, K, c->vht);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> tot(2 * (size_t)c->V);
+    HIPCHK(c, hipMemcpyAsync(hs.data(), c->st, sizeof(VarState) * c->V, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(tot.data(), c->vht, sizeof(double) * 2 * c->V, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int v = 0; v < c->V; ++v) {
       tapgpu_result& r = results[v];
       r.iterations = hs[v].iter;
       r.relgap = hs[v].relgap;
       r.objective = hs[v].objective;
       r.converged = (target_relgap > 0 && hs[v].relgap <= target_relgap) ? 1 : 0;
-      // VHT after the last step: sum x*c over present links (host, fixed order)
-      HIPCHK(c, hipMemcpy(x.data(), c->x + (size_t)v * c->E, sizeof(double) * c->E,
-                          hipMemcpyDeviceToHost));
-      HIPCHK(c, hipMemcpy(cc.data(), c->c + (size_t)v * c->E, sizeof(double) * c->E,
-                          hipMemcpyDeviceToHost));
-      tap_acc a{0.0, 0.0};
-      for (int k = 0; k < c->E; ++k) tap_acc_add(&a, x[k] * cc[k]);
-      r.vht = tap_acc_value(a);
+      r.vht = tot[2 * (size_t)v];
     }
   }
   return TAPGPU_OK;
@@ -3035,6 +3401,46 @@ int tapgpu_get_flows(tapgpu_ctx* c, int64_t variant, double* volumes, double* co
   if (costs)
     HIPCHK(c, hipMemcpy(costs, c->c + (size_t)variant * c->E, sizeof(double) * c->E,
                         hipMemcpyDeviceToHost));
+  return TAPGPU_OK;
+}
+
+int tapgpu_get_totals(tapgpu_ctx* c, int64_t variant, double* objective, double* vht) {
+  if (!c) return TAPGPU_E_ARG;
+  int rc = check_variant(c, variant);
+  if (rc) return rc;
+  if (!c->initialised) return c->fail(TAPGPU_E_STATE, "tapgpu_fw_init was not called");
+  if (!c->vht && (rc = dalloc(c, &c->vht, 2 * (size_t)c->V))) return rc;
+  hipLaunchKernelGGL(k_totals, dim3(c->V), dim3(kBlock), 0, c->stream, make_links(c), c->vht);
+  HIPCHK(c, hipGetLastError());
+  double t[2];
+  HIPCHK(c, hipMemcpyAsync(t, c->vht + 2 * variant, sizeof t, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (vht) *vht = t[0];
+  if (objective) *objective = t[1];
+  return TAPGPU_OK;
+}
+
+int tapgpu_set_links(tapgpu_ctx* c, int64_t variant, const double* capacity,
+                     const double* free_flow_time) {
+  if (!c) return TAPGPU_E_ARG;
+  int rc = check_variant(c, variant);
+  if (rc) return rc;
+  const size_t off = (size_t)variant * c->E;
+  if (capacity) {
+    std::vector<unsigned char> pres(c->E);
+    HIPCHK(c, hipMemcpy(pres.data(), c->present + off, c->E, hipMemcpyDeviceToHost));
+    std::vector<double> cap(capacity, capacity + c->E);
+    double minpos = DBL_MAX;
+    for (int64_t k = 0; k < c->E; ++k)
+      if (pres[k] && cap[k] > 0.0 && cap[k] < minpos) minpos = cap[k];
+    for (auto& v : cap)
+      if (!(v > 0.0)) v = minpos;
+    HIPCHK(c, hipMemcpy(c->cap + off, cap.data(), sizeof(double) * c->E, hipMemcpyHostToDevice));
+  }
+  if (free_flow_time)
+    HIPCHK(c, hipMemcpy(c->t0 + off, free_flow_time, sizeof(double) * c->E, hipMemcpyHostToDevice));
+  // the volumes no longer belong to these links: fw_init comes next
+  c->initialised = false;
   return TAPGPU_OK;
 }
 
@@ -3115,7 +3521,9 @@ int tapgpu_get_stats(tapgpu_ctx* c, tapgpu_stats* s) {
 }
 
 int tapgpu_set_strategy(tapgpu_ctx* c, int strategy) {
-  if (!c || strategy < 0 || strategy > 3) return TAPGPU_E_ARG;
+  if (!c || strategy < 0 || strategy > 4) return TAPGPU_E_ARG;
+  if (strategy == 4 && !(c->multi_ok && c->hbm_ok))
+    return c->fail(TAPGPU_E_ARG, "frontier bitmaps of this network do not fit the multi-origin kernel");
   if (strategy == 2 && !c->lds_ok)
     return c->fail(TAPGPU_E_ARG, "network does not fit the one-origin-per-workgroup path");
   if (strategy == 3 && !c->hbm_ok)
@@ -3146,6 +3554,27 @@ int tapgpu_set_origin_split(tapgpu_ctx* c, int origins) {
 
 int tapgpu_get_origin_split(const tapgpu_ctx* c) { return c ? c->split : TAPGPU_E_ARG; }
 
+int tapgpu_set_origin_block(tapgpu_ctx* c, int threads) {
+  if (!c) return TAPGPU_E_ARG;
+  if (threads == -1) {
+    c->lds_block = -1;
+    return TAPGPU_OK;
+  }
+  bool known = false;
+  for (int b : kLdsBlocks) known = known || b == threads;
+  if (!known) return c->fail(TAPGPU_E_ARG, "workgroup size %d: 1024, 512, 256 or -1", threads);
+  if (c->N > kLdsPerThread * threads)
+    return c->fail(TAPGPU_E_ARG, "%d threads per workgroup hold at most %d nodes", threads,
+                   kLdsPerThread * threads);
+  c->lds_block = threads;
+  return TAPGPU_OK;
+}
+
+int tapgpu_get_origin_block(const tapgpu_ctx* c) {
+  if (!c) return TAPGPU_E_ARG;
+  return c->lds_block;
+}
+
 int tapgpu_get_origin_chase(const tapgpu_ctx* c) {
   if (!c) return TAPGPU_E_ARG;
   return c->chase;
@@ -3154,6 +3583,8 @@ int tapgpu_get_origin_chase(const tapgpu_ctx* c) {
 int tapgpu_set_origin_queue(tapgpu_ctx* c, int entries) {
   if (!c || entries < -1) return TAPGPU_E_ARG;
   c->qcap = entries < 0 ? c->qcap_max : std::min(entries, c->qcap_max);
+  c->qcap_auto = entries < 0;
+  if (entries < 0) c->lds_block = -1;     // picked again with the workgroup size
   c->hqcap = entries < 0 ? c->hqcap_max : std::min(entries, c->hqcap_max);
   c->lanes_qcap = entries < 0 ? c->lanes_qcap_max : std::min(entries, c->lanes_qcap_max);
   return TAPGPU_OK;
@@ -3168,7 +3599,7 @@ int tapgpu_get_origin_queue(const tapgpu_ctx* c) {
 int tapgpu_get_strategy(const tapgpu_ctx* c) {
   if (!c) return TAPGPU_E_ARG;
   tapgpu_ctx* m = const_cast<tapgpu_ctx*>(c);
-  return use_lds(m) ? 2 : (use_hbm(m) ? 3 : 1);
+  return use_multi(m) ? 4 : (use_lds(m) ? 2 : (use_hbm(m) ? 3 : 1));
 }
 
 int tapgpu_set_lanes(tapgpu_ctx* c, int min_active) {
@@ -3180,6 +3611,14 @@ int tapgpu_set_lanes(tapgpu_ctx* c, int min_active) {
 }
 
 int tapgpu_get_lanes(const tapgpu_ctx* c) { return c ? c->lanes_min : TAPGPU_E_ARG; }
+
+int tapgpu_set_lanes_delta(tapgpu_ctx* c, double mult) {
+  if (!c || !(mult >= -1.0) || !std::isfinite(mult)) return TAPGPU_E_ARG;
+  c->lanes_delta_mult = mult < 0 ? -1.0 : mult;
+  return TAPGPU_OK;
+}
+
+double tapgpu_get_lanes_delta(const tapgpu_ctx* c) { return c ? c->lanes_delta_mult : -2.0; }
 
 int tapgpu_set_profiling(tapgpu_ctx* c, int on) {
   if (!c) return TAPGPU_E_ARG;

@@ -71,8 +71,12 @@ class _Stats(C.Structure):
                 ("origin_load_ms", C.c_double), ("origin_tune_ms", C.c_double),
                 ("origin_chase", C.c_int64), ("origin_delta", C.c_double),
                 ("origin_split", C.c_int64), ("origin_deferred", C.c_int64),
+                ("origin_block", C.c_int64), ("origin_queue", C.c_int64),
                 ("lanes_launches", C.c_int64), ("lanes_tasks", C.c_int64),
-                ("lanes_pred_scans", C.c_int64), ("lanes_fallbacks", C.c_int64)]
+                ("lanes_pred_scans", C.c_int64), ("lanes_fallbacks", C.c_int64),
+                ("lanes_delta", C.c_double), ("graph_launches", C.c_int64),
+                ("multi_launches", C.c_int64), ("multi_fallbacks", C.c_int64),
+                ("multi_k", C.c_int64)]
 
 
 class _GreedyResult(C.Structure):
@@ -124,8 +128,19 @@ def lib():
         L.tapgpu_get_origin_queue.argtypes = [vp]
         L.tapgpu_set_origin_delta.argtypes = [vp, C.c_double]
         L.tapgpu_set_origin_split.argtypes = [vp, C.c_int]
+        L.tapgpu_set_origin_block.argtypes = [vp, C.c_int]
+        L.tapgpu_get_origin_block.argtypes = [vp]
         L.tapgpu_set_lanes.argtypes = [vp, C.c_int]
         L.tapgpu_get_lanes.argtypes = [vp]
+        L.tapgpu_set_lanes_delta.argtypes = [vp, C.c_double]
+        L.tapgpu_get_lanes_delta.argtypes = [vp]
+        L.tapgpu_get_lanes_delta.restype = C.c_double
+        L.tapgpu_get_totals.argtypes = [vp, C.c_int64, vp, vp]
+        L.tapgpu_set_links.argtypes = [vp, C.c_int64, vp, vp]
+        L.tapgpu_set_multi.argtypes = [vp, C.c_int, C.c_double]
+        L.tapgpu_get_multi.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.tapgpu_set_graphs.argtypes = [vp, C.c_int]
+        L.tapgpu_get_graphs.argtypes = [vp, C.POINTER(C.c_int)]
         L.tapgpu_greedy.argtypes = [vp, C.c_int64, C.c_int, vp, vp, vp, vp, C.c_int,
                                     C.POINTER(_GreedyResult)]
         L.tapgpu_get_origin_split.argtypes = [vp]
@@ -260,6 +275,22 @@ class TapEngine:
         self._check(lib().tapgpu_get_flows(self._h, variant, _p(x), _p(c)))
         return x, c
 
+    def totals(self, variant=0):
+        """(Beckmann objective, VHT) at the variant's current volumes."""
+        o, v = C.c_double(), C.c_double()
+        self._check(lib().tapgpu_get_totals(self._h, variant, C.byref(o), C.byref(v)))
+        return o.value, v.value
+
+    def set_links(self, capacity=None, fftime=None, variant=0):
+        """Replace the variant's capacities and/or free-flow times (fw_init
+        must follow)."""
+        a = None if capacity is None else np.ascontiguousarray(capacity, dtype=np.float64)
+        b = None if fftime is None else np.ascontiguousarray(fftime, dtype=np.float64)
+        for t in (a, b):
+            if t is not None and t.shape != (self.E,):
+                raise ValueError("link arrays must have num_links entries")
+        self._check(lib().tapgpu_set_links(self._h, variant, _p(a), _p(b)))
+
     def trace(self, variant=0):
         n = C.c_int(0)
         self._check(lib().tapgpu_get_trace(self._h, variant, None, 0, C.byref(n)))
@@ -301,14 +332,14 @@ class TapEngine:
                     unsatisfied=res.unsatisfied, total_cost=res.total_cost,
                     unsat_trace=un[:n], ms_trace=ms[:n])
 
-    STRATEGIES = {"auto": 0, "sweeps": 1, "lds": 2, "hbm": 3}
+    STRATEGIES = {"auto": 0, "sweeps": 1, "lds": 2, "hbm": 3, "multi": 4}
 
     def set_strategy(self, name):
         self._check(lib().tapgpu_set_strategy(self._h, self.STRATEGIES[name]))
 
     @property
     def strategy(self):
-        return {1: "sweeps", 2: "lds", 3: "hbm"}[lib().tapgpu_get_strategy(self._h)]
+        return {1: "sweeps", 2: "lds", 3: "hbm", 4: "multi"}[lib().tapgpu_get_strategy(self._h)]
 
     def set_origin_chase(self, chase):
         """Chained expansions per thread and round on the one-origin-per-
@@ -339,6 +370,15 @@ class TapEngine:
     def origin_split(self):
         return lib().tapgpu_get_origin_split(self._h)
 
+    def set_origin_block(self, threads):
+        """Workgroup size of the LDS path (1024, 512, 256; -1: tune with the
+        queue size on the next loading)."""
+        self._check(lib().tapgpu_set_origin_block(self._h, int(threads)))
+
+    @property
+    def origin_block(self):
+        return lib().tapgpu_get_origin_block(self._h)
+
     def set_origin_delta(self, mult):
         """Bucket width of the labels-in-HBM path in mean free-flow link
         times (0: none, -1: tune on the next loading)."""
@@ -356,6 +396,37 @@ class TapEngine:
     @property
     def lanes(self):
         return lib().tapgpu_get_lanes(self._h)
+
+    def set_lanes_delta(self, mult):
+        """Bucket width of the lanes kernel in mean free-flow link times (0:
+        none, -1: tune on the next loading)."""
+        self._check(lib().tapgpu_set_lanes_delta(self._h, float(mult)))
+
+    @property
+    def lanes_delta(self):
+        return lib().tapgpu_get_lanes_delta(self._h)
+
+    def set_multi(self, origins=-1, delta=-1.0):
+        """Strategy 4's origins per workgroup (4, 8, 16) and bucket width
+        (mean free-flow times; -1: tune on the next loading)."""
+        self._check(lib().tapgpu_set_multi(self._h, int(origins), float(delta)))
+
+    @property
+    def multi(self):
+        k, d = C.c_int(), C.c_double()
+        self._check(lib().tapgpu_get_multi(self._h, C.byref(k), C.byref(d)))
+        return k.value, d.value
+
+    def set_graphs(self, on=True):
+        """Replay each Frank-Wolfe iteration from a captured HIP graph (on,
+        the default) or issue its launches one by one."""
+        self._check(lib().tapgpu_set_graphs(self._h, int(bool(on))))
+
+    @property
+    def graphs(self):
+        v = C.c_int()
+        self._check(lib().tapgpu_get_graphs(self._h, C.byref(v)))
+        return bool(v.value)
 
     def set_profiling(self, on=True):
         self._check(lib().tapgpu_set_profiling(self._h, int(bool(on))))

@@ -22,6 +22,13 @@ errors of the order of sqrt(g)..g; the tests pin the kernels at equilibria
 solved to rounding level.  `eps` defaults to max(1e-9, 20 * relgap reached).
 x is returned without a gradient (its Jacobian is not formed).
 
+One engine (topology, demand and schedule resident on the GPU) serves
+every call on the same network and demand: the arrays are compared with
+the ones it was built from, and each call only replaces the link capacities
+and free-flow times (tapgpu_set_links).  The objective and VHT come from the
+engine (tapgpu_get_totals: the reference's link cost integral, compensated
+sums).
+
 There is no CPU fallback: the engine library and a GPU are required.
 """
 from __future__ import annotations
@@ -36,6 +43,43 @@ def _np(t):
     return t.detach().to("cpu", torch.float64).numpy()
 
 
+class _EngineCache:
+    """The engine of the last (network, demand) seen, kept while their
+    topology, BPR shapes and demand are unchanged (compared array by array,
+    so arrays changed in place are seen)."""
+
+    def __init__(self):
+        self.key = None
+        self.eng = None
+        self.builds = 0
+
+    @staticmethod
+    def _key(net, dem):
+        return ((net.num_nodes, net.num_zones, net.first_thru_node, dem.num_zones),
+                tuple(np.array(a, copy=True) for a in (net.init, net.term, net.B, net.power,
+                                                        dem.start, dem.dest, dem.demand)))
+
+    def get(self, net, dem):
+        key = self._key(net, dem)
+        if self.eng is not None and key[0] == self.key[0] and all(
+                a.shape == b.shape and np.array_equal(a, b) for a, b in zip(key[1], self.key[1])):
+            return self.eng
+        self.close()
+        self.eng = TapEngine(net, dem)
+        self.key = key
+        self.builds += 1
+        return self.eng
+
+    def close(self):
+        if self.eng is not None:
+            self.eng.close()
+        self.eng = None
+        self.key = None
+
+
+_CACHE = _EngineCache()
+
+
 class EquilibriumFunction(torch.autograd.Function):
     """(capacity, fftime) -> (Beckmann objective, VHT, volumes) at equilibrium."""
 
@@ -46,29 +90,23 @@ class EquilibriumFunction(torch.autograd.Function):
         t0 = _np(fftime)
         if cap.shape != (net.num_links,) or t0.shape != (net.num_links,):
             raise ValueError("capacity and fftime must have num_links entries")
-        eng = TapEngine(net, dem, num_variants=1, capacity=cap[None, :], fftime=t0[None, :])
+        if max_iters <= 0 and (warm_volumes is None or eps is None):
+            raise ValueError("max_iters=0 needs warm_volumes and eps")
+        eng = _CACHE.get(net, dem)
         try:
+            eng.set_links(cap, t0)
             eng.fw_init(None if warm_volumes is None else np.asarray(warm_volumes, np.float64))
             if max_iters > 0:
                 res = eng.fw_run(max_iters, target_relgap)[0]
             else:           # evaluate at the warm-start volumes as given
-                if warm_volumes is None or eps is None:
-                    raise ValueError("max_iters=0 needs warm_volumes and eps")
                 res = dict(iterations=0, converged=False, relgap=float("nan"))
-            x, c = eng.flows()
+            x, _ = eng.flows()
+            obj, vht = eng.totals()
             e = eps if eps is not None else max(1e-9, 20.0 * max(res["relgap"], 0.0))
             g = eng.gradient(0, eps=e, cg_tol=cg_tol)
-        finally:
-            eng.close()
-        B = np.asarray(net.B, np.float64)
-        p = np.asarray(net.power, np.float64)
-        capf = cap.copy()
-        pos = capf[capf > 0]
-        if pos.size:
-            capf[capf <= 0] = pos.min()
-        q = x / capf
-        obj = float(np.sum(t0 * x * (1.0 + B * q ** p / (p + 1.0))))
-        vht = float(np.dot(x, c))
+        except BaseException:
+            _CACHE.close()          # state unknown after an error: rebuild next time
+            raise
         dev = capacity.device
         ctx.save_for_backward(*(torch.from_numpy(g[k]).to(dev)
                                 for k in ("dobj_dcap", "dobj_dt0", "dvht_dcap", "dvht_dt0")))
