@@ -244,10 +244,6 @@ typedef struct {
   int64_t lanes_fallbacks;      /* loadings redone by a per-origin kernel    */
   double lanes_delta;           /* its bucket width in mean free-flow times  */
   int64_t graph_launches;       /* iterations replayed from captured graphs  */
-  /* multi-origin kernel (strategy 4) */
-  int64_t multi_launches;       /* launches                                  */
-  int64_t multi_fallbacks;      /* loadings redone by the labels-in-HBM kernel */
-  int64_t multi_k;              /* origins per workgroup                     */
 } tapgpu_stats;
 
 /* Counters and wall times of the last tapgpu_fw_run (or of the calls since
@@ -262,18 +258,8 @@ int tapgpu_set_profiling(tapgpu_ctx *ctx, int on);
    origin per workgroup with labels in LDS (networks whose labels fit a
    workgroup's LDS: about 13k nodes, fewer than 65535 links), 3 = one origin
    per workgroup with labels in HBM (frontier bitmaps in LDS: up to about
-   430k nodes), 4 = K neighbouring origins per workgroup with their labels
-   side by side in HBM (loadings; shortest-path-only calls and loadings with
-   zero-cost level ties use 3).  tapgpu_get_strategy returns the one in use
-   (1, 2, 3 or 4). */
+   430k nodes).  tapgpu_get_strategy returns the one in use (1, 2 or 3). */
 int tapgpu_set_strategy(tapgpu_ctx *ctx, int strategy);
-
-/* Schedule of strategy 4: origins per workgroup (4, 8 or 16) and the bucket
-   width of its label-ordered rounds in mean free-flow link times (0: none).
-   -1 (the default for either): tuned on the next loading by timing each
-   candidate on the real launch.  Results do not depend on them. */
-int tapgpu_set_multi(tapgpu_ctx *ctx, int origins, double delta_mult);
-int tapgpu_get_multi(const tapgpu_ctx *ctx, int *origins, double *delta_mult);
 
 /* Worklist schedule of the one-origin-per-workgroup path (strategy 2),
    rounds that scan the frontier bitmap (see tapgpu_set_origin_queue): how

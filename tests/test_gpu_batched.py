@@ -17,7 +17,7 @@ def _small():
     return base, dem, mods, subsets
 
 
-@pytest.mark.parametrize("strategy", ["sweeps", "lds", "hbm", "lanes", "multi"])
+@pytest.mark.parametrize("strategy", ["sweeps", "lds", "hbm", "lanes"])
 def test_batched_equals_plain_and_oracle(engine_mod, oracle_lib, strategy):
     """Every loading path; "lanes" puts the 12 variants in the lanes of one
     group (the per-origin kernel takes over once fewer than 4 are active)."""

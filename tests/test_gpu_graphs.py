@@ -6,9 +6,8 @@ Each run goes in chunks of a few iterations with the program's own eager
 work between the replays (volume readback, an extra loading, synchronising
 copies), then to relgap 1e-4 (at most 400 iterations); both runs are compared after every chunk and
 over every iteration's trace record.  Cases: the per-origin LDS and
-labels-in-HBM kernels (whole iteration captured), the node sweeps, the
-LDS kernel with HBM co-run origins and the multi-origin kernel (link passes
-captured), the lanes kernel
+labels-in-HBM kernels (whole iteration captured), the node sweeps and the
+LDS kernel with HBM co-run origins (link passes captured), the lanes kernel
 on one variant and on a 12-variant batch (whole iteration captured), and
 the lanes kernel on a network with zero-cost links, whose level ties stop
 the captured step and send every loading to a per-origin kernel."""
@@ -20,7 +19,7 @@ import pytest
 from conftest import network, random_demand, random_network
 
 pytestmark = pytest.mark.gpu
-CASES = ["lds", "hbm", "sweeps", "lds-split", "multi", "lanes", "lanes-batch", "lanes-ties"]
+CASES = ["lds", "hbm", "sweeps", "lds-split", "lanes", "lanes-batch", "lanes-ties"]
 
 
 def _engine(engine_mod, case):

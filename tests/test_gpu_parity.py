@@ -15,10 +15,9 @@ from conftest import (b64_array, empty_demand, golden, hexarr, network, pred_rea
 pytestmark = pytest.mark.gpu
 TOL = 1e-9
 STRATS = ["sweeps", "lds", "hbm"]
-# loading paths: the three strategies, the lanes kernel (variants in
-# wavefront lanes; here one variant, lane 0 of its group) and the
-# multi-origin kernel (neighbouring origins in wavefront lanes)
-LOAD_STRATS = STRATS + ["lanes", "multi"]
+# loading paths: the three strategies plus the lanes kernel (variants in
+# wavefront lanes; here one variant, lane 0 of its group)
+LOAD_STRATS = STRATS + ["lanes"]
 
 
 def make_engine(engine_mod, net, dem, strategy, **kw):
@@ -346,7 +345,7 @@ def test_fw_relgap_target_and_resume(engine_mod, oracle_lib, strategy):
     assert o["iterations"] == r1["iterations"]
 
 
-@pytest.mark.parametrize("strategy", ["hbm", "sweeps", "multi"])
+@pytest.mark.parametrize("strategy", ["hbm", "sweeps"])
 def test_fw_regional_first_iterations(engine_mod, oracle_lib, strategy):
     """The regional network (94k nodes, 4,000 zones; too large for LDS
     labels): two FW iterations bit-identical to the oracle (correctly
@@ -454,40 +453,6 @@ def test_schedule_knobs_do_not_change_results(engine_mod):
         eng.set_origin_block(-1)
         eng.aon()
         assert eng.origin_block == 1024
-
-
-@pytest.mark.parametrize("scale", [0.25, 1.0])
-def test_multi_origin_schedules(engine_mod, scale):
-    """Strategy 4 (K neighbouring origins per workgroup) on the regional
-    network: volumes bit-identical to the labels-in-HBM kernel's for every
-    K and bucket width, at free-flow and at perturbed costs; bad values
-    refused."""
-    print(f"case multi schedules regional@{scale}", flush=True)
-    net, dem = network("regional", scale)
-    rng = np.random.default_rng(3)
-    costs = net.fftime * rng.uniform(1.0, 3.0, net.num_links)
-    with engine_mod.TapEngine(net, dem) as eng:
-        eng.set_strategy("hbm")
-        ref = [eng.aon()]
-        eng.set_costs(costs)
-        ref.append(eng.aon())
-        eng.set_strategy("multi")
-        assert eng.strategy == "multi"
-        n0 = eng.stats()["multi_launches"]       # (stats add up over loadings)
-        for i, (k, d) in enumerate(((4, 0.0), (8, 2.0), (16, 4.0), (8, 0.0), (-1, -1.0))):
-            eng.set_multi(k, d)
-            eng.set_costs(net.fftime)
-            y0 = eng.aon()
-            eng.set_costs(costs)
-            y1 = eng.aon()
-            st = eng.stats()
-            assert st["multi_launches"] == n0 + 2 * (i + 1) and st["multi_fallbacks"] == 0
-            assert np.array_equal(y0, ref[0]) and np.array_equal(y1, ref[1]), (k, d)
-        k, d = eng.multi
-        assert k in (4, 8, 16) and d >= 0
-        for bad in ((5, 0.0), (8, float("nan")), (0, 1.0)):
-            with pytest.raises(engine_mod.TapGpuError):
-                eng.set_multi(*bad)
 
 
 def test_fw_warm_start_continues(engine_mod):

@@ -1610,15 +1610,6 @@ struct tapgpu_ctx {
   double *l_W = nullptr, *l_Y = nullptr;                     // [groups][E][64]
   int* l_lane_var = nullptr;          // [groups * 64]
   std::vector<int> h_lane_var;        // its host copy (as uploaded)
-  // multi-origin kernel (strategy 4)
-  bool multi_ok = false;              // its LDS (bitmaps + queues) fits
-  int multi_qcap = 0, multi_qcap_max = 0;
-  size_t multi_lds = 0;
-  int multi_grid = 0, multi_grid_max = 0;
-  int multi_k = -1;                   // origins per task (-1: tune)
-  double multi_delta_mult = -1;       // bucket width / mean free-flow time (-1: tune)
-  unsigned long long* m_dist = nullptr;
-  int *m_pred = nullptr, *m_ptail = nullptr, *m_zones = nullptr;
   unsigned l_stamp = 1;               // stamp of the next task's memo entries
   unsigned* l_stamp_dev = nullptr;    // its device copy, read by the kernel
   bool capturing = false;             // launches are being recorded into a graph
@@ -1865,12 +1856,8 @@ bool use_lds(tapgpu_ctx* c) {
 
 // auto: labels in LDS where they fit, else labels in HBM
 bool use_hbm(tapgpu_ctx* c) {
-  return c->hbm_ok && (c->strategy == 3 || c->strategy == 4 || (c->strategy == 0 && !c->lds_ok));
+  return c->hbm_ok && (c->strategy == 3 || (c->strategy == 0 && !c->lds_ok));
 }
-
-// strategy 4: loadings by the multi-origin kernel (shortest-path-only calls
-// and its level-tie fallback use the labels-in-HBM kernel)
-bool use_multi(tapgpu_ctx* c) { return c->multi_ok && c->strategy == 4; }
 
 OriginArgs make_origin_args(tapgpu_ctx* c, const unsigned char* var_on) {
   OriginArgs A{};
@@ -2232,7 +2219,6 @@ int aon_hbm(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
 }
 
 #include "taplanes.inc"
-#include "tapmulti.inc"
 
 // The lanes kernel loads a batch when at least lanes_min of its variants
 // are active (lanes_min 0: never).
@@ -2266,12 +2252,6 @@ int redo_per_origin(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) 
 
 int load_variants(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on,
                   const unsigned char* h_on) {
-  if (use_multi(c) && !use_lanes(c, v0, Vn, h_on)) {
-    bool fallback = false;
-    int rc = aon_multi(c, v0, Vn, var_on, &fallback);
-    if (rc || !fallback) return rc;
-    return redo_per_origin(c, v0, Vn, var_on);      // (strategy 4 redoes with k_origin_hbm)
-  }
   if (use_lanes(c, v0, Vn, h_on)) {
     bool fallback = false;
     int rc = aon_lanes(c, v0, Vn, var_on, h_on, &fallback);
@@ -2514,34 +2494,6 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
         c->lanes_grid_max = std::max(1, ncu * pc);
       }
       c->lanes_min = V > 1 ? kLanesMinDefault : 0;
-    }
-    // multi-origin kernel: three bitmaps, then queues from the LDS share of
-    // the kMultiPerCu workgroups on a CU
-    {
-      int lds_cu = 0;
-      if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
-              hipSuccess || lds_cu <= 0)
-        lds_cu = lds_max;
-      const size_t base = multi_lds_bytes(c->N, 0);
-      const size_t share = std::min<size_t>((size_t)lds_cu / kMultiPerCu, (size_t)lds_max) - 256;
-      c->multi_ok = base + 64 * 24 <= share;
-      if (c->multi_ok) {
-        c->multi_qcap_max = (int)std::min<size_t>(kMultiQueueMax, (share - base) / 24) & ~63;
-        c->multi_qcap = c->multi_qcap_max;
-        c->multi_lds = multi_lds_bytes(c->N, c->multi_qcap_max);
-        int pc = 0;
-        const void* fns[] = {reinterpret_cast<const void*>(k_origin_multi<4>),
-                             reinterpret_cast<const void*>(k_origin_multi<8>),
-                             reinterpret_cast<const void*>(k_origin_multi<16>)};
-        bool ok = true;
-        for (const void* f : fns)
-          ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)c->multi_lds) == hipSuccess;
-        if (!ok || hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k_origin_multi<8>, kMultiBlock,
-                                                                 c->multi_lds) != hipSuccess || pc < 1)
-          pc = 1;
-        c->multi_grid_max = std::max(1, ncu * pc);
-      }
     }
     double sum = 0;
     int cnt = 0;
@@ -3256,10 +3208,8 @@ int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_res
     auto t_it = std::chrono::steady_clock::now();
     // with the schedule already tuned, one kernel opens the iteration and
     // clears what the loading would clear
-    const bool multi_now = use_multi(c);
-    const bool tuned = multi_now ? (c->multi_k > 0 && c->multi_delta_mult >= 0)
-                       : use_lds(c) ? (c->chase >= 0 && c->split >= 0 && c->lds_block > 0)
-                                    : (use_hbm(c) && c->delta_mult >= 0);
+    const bool tuned = use_lds(c) ? (c->chase >= 0 && c->split >= 0 && c->lds_block > 0)
+                                  : (use_hbm(c) && c->delta_mult >= 0);
     const bool graphs = c->graphs && !c->profiling;
     const bool lanes_now = use_lanes(c, 0, c->V, hon.data());
     float aon = 0.f;
@@ -3271,8 +3221,7 @@ int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_res
     if (graphs && lanes_now && c->lanes_delta_mult >= 0 && c->l_dist && c->Z > 0) {
       if ((rc = lanes_map(c, 0, c->V, hon.data(), &G))) return rc;
       if (G > 0) mode = 2;
-    } else if (graphs && tuned && !lanes_now && !multi_now &&
-               (use_lds(c) ? c->split == 0 : use_hbm(c))) {
+    } else if (graphs && tuned && !lanes_now && (use_lds(c) ? c->split == 0 : use_hbm(c))) {
       mode = 1;
     }
     if (mode == 1) {
@@ -3521,9 +3470,7 @@ int tapgpu_get_stats(tapgpu_ctx* c, tapgpu_stats* s) {
 }
 
 int tapgpu_set_strategy(tapgpu_ctx* c, int strategy) {
-  if (!c || strategy < 0 || strategy > 4) return TAPGPU_E_ARG;
-  if (strategy == 4 && !(c->multi_ok && c->hbm_ok))
-    return c->fail(TAPGPU_E_ARG, "frontier bitmaps of this network do not fit the multi-origin kernel");
+  if (!c || strategy < 0 || strategy > 3) return TAPGPU_E_ARG;
   if (strategy == 2 && !c->lds_ok)
     return c->fail(TAPGPU_E_ARG, "network does not fit the one-origin-per-workgroup path");
   if (strategy == 3 && !c->hbm_ok)
@@ -3599,7 +3546,7 @@ int tapgpu_get_origin_queue(const tapgpu_ctx* c) {
 int tapgpu_get_strategy(const tapgpu_ctx* c) {
   if (!c) return TAPGPU_E_ARG;
   tapgpu_ctx* m = const_cast<tapgpu_ctx*>(c);
-  return use_multi(m) ? 4 : (use_lds(m) ? 2 : (use_hbm(m) ? 3 : 1));
+  return use_lds(m) ? 2 : (use_hbm(m) ? 3 : 1);
 }
 
 int tapgpu_set_lanes(tapgpu_ctx* c, int min_active) {

@@ -74,9 +74,7 @@ class _Stats(C.Structure):
                 ("origin_block", C.c_int64), ("origin_queue", C.c_int64),
                 ("lanes_launches", C.c_int64), ("lanes_tasks", C.c_int64),
                 ("lanes_pred_scans", C.c_int64), ("lanes_fallbacks", C.c_int64),
-                ("lanes_delta", C.c_double), ("graph_launches", C.c_int64),
-                ("multi_launches", C.c_int64), ("multi_fallbacks", C.c_int64),
-                ("multi_k", C.c_int64)]
+                ("lanes_delta", C.c_double), ("graph_launches", C.c_int64)]
 
 
 class _GreedyResult(C.Structure):
@@ -137,8 +135,6 @@ def lib():
         L.tapgpu_get_lanes_delta.restype = C.c_double
         L.tapgpu_get_totals.argtypes = [vp, C.c_int64, vp, vp]
         L.tapgpu_set_links.argtypes = [vp, C.c_int64, vp, vp]
-        L.tapgpu_set_multi.argtypes = [vp, C.c_int, C.c_double]
-        L.tapgpu_get_multi.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.tapgpu_set_graphs.argtypes = [vp, C.c_int]
         L.tapgpu_get_graphs.argtypes = [vp, C.POINTER(C.c_int)]
         L.tapgpu_greedy.argtypes = [vp, C.c_int64, C.c_int, vp, vp, vp, vp, C.c_int,
@@ -332,14 +328,14 @@ class TapEngine:
                     unsatisfied=res.unsatisfied, total_cost=res.total_cost,
                     unsat_trace=un[:n], ms_trace=ms[:n])
 
-    STRATEGIES = {"auto": 0, "sweeps": 1, "lds": 2, "hbm": 3, "multi": 4}
+    STRATEGIES = {"auto": 0, "sweeps": 1, "lds": 2, "hbm": 3}
 
     def set_strategy(self, name):
         self._check(lib().tapgpu_set_strategy(self._h, self.STRATEGIES[name]))
 
     @property
     def strategy(self):
-        return {1: "sweeps", 2: "lds", 3: "hbm", 4: "multi"}[lib().tapgpu_get_strategy(self._h)]
+        return {1: "sweeps", 2: "lds", 3: "hbm"}[lib().tapgpu_get_strategy(self._h)]
 
     def set_origin_chase(self, chase):
         """Chained expansions per thread and round on the one-origin-per-
@@ -405,17 +401,6 @@ class TapEngine:
     @property
     def lanes_delta(self):
         return lib().tapgpu_get_lanes_delta(self._h)
-
-    def set_multi(self, origins=-1, delta=-1.0):
-        """Strategy 4's origins per workgroup (4, 8, 16) and bucket width
-        (mean free-flow times; -1: tune on the next loading)."""
-        self._check(lib().tapgpu_set_multi(self._h, int(origins), float(delta)))
-
-    @property
-    def multi(self):
-        k, d = C.c_int(), C.c_double()
-        self._check(lib().tapgpu_get_multi(self._h, C.byref(k), C.byref(d)))
-        return k.value, d.value
 
     def set_graphs(self, on=True):
         """Replay each Frank-Wolfe iteration from a captured HIP graph (on,
