@@ -238,6 +238,11 @@ def regional_sssp(args):
     return roof
 
 
+def eng_can_block(eng, block):
+    """A fixed LDS workgroup size applies to the LDS path only."""
+    return block > 0 and eng.strategy == "lds"
+
+
 def bench_batched(args):
     """Batched form: --variants upgrade subsets of the metro network in one
     engine context.  K timed steps = K Frank-Wolfe iterations of the whole
@@ -253,6 +258,7 @@ def bench_batched(args):
         eng.set_strategy(args.strategy)
     if args.lanes is not None:
         eng.set_lanes(args.lanes)
+    eng.set_lanes_delta(args.lanes_delta)
     # warmup: the initial loading and W iterations (schedule tuning included)
     eng.fw_init()
     eng.fw_run(max_iters=args.warmup, target_relgap=TARGET)
@@ -431,6 +437,8 @@ def main():
                     help="batched: variants solved one by one for the plain comparison")
     ap.add_argument("--lanes", type=int, default=None,
                     help="batched: lanes kernel from this many active variants (0: off)")
+    ap.add_argument("--lanes-delta", type=float, default=-1.0,
+                    help="batched: lanes kernel bucket width in mean free-flow times (-1: tuned)")
     ap.add_argument("--no-solve", action="store_true",
                     help="batched: time the K steps only (profiling runs)")
     ap.add_argument("--no-regional", action="store_true",
@@ -441,6 +449,8 @@ def main():
     ap.add_argument("--strategy", default="auto", choices=["auto", "sweeps", "lds", "hbm"])
     ap.add_argument("--chase", type=int, default=-1,
                     help="LDS path: chained expansions per bitmap-scan round (-1: tuned)")
+    ap.add_argument("--block", type=int, default=-1,
+                    help="LDS path: threads per workgroup, 1024/512/256 (-1: tuned with the queue size)")
     ap.add_argument("--delta", type=float, default=-1.0,
                     help="HBM path: bucket width in mean free-flow times (-1: tuned)")
     ap.add_argument("--max-iters", type=int, default=0,
@@ -467,6 +477,8 @@ def main():
         eng.set_strategy(args.strategy)
     eng.set_origin_chase(args.chase)
     eng.set_origin_delta(args.delta)
+    if eng_can_block(eng, args.block):
+        eng.set_origin_block(args.block)
     for _ in range(args.warmup):
         eng.fw_init()
         eng.fw_run(max_iters=args.max_iters, target_relgap=TARGET)
