@@ -330,15 +330,19 @@ int tapgpu_set_lanes_delta(tapgpu_ctx *ctx, double mult);
 double tapgpu_get_lanes_delta(const tapgpu_ctx *ctx);
 
 /* Launch form of tapgpu_fw_run's iterations (no counterpart in the
-   reference, whose steps are host loops).  on = 1 (the default): once the
-   per-origin schedule is tuned, each iteration (opening kernel, loading,
-   both line-search windows, step, readback) is one replay of a captured HIP
-   graph; with the lanes kernel the link passes are.  on = 0: the same
-   launches issued one by one.  Profiling (tapgpu_set_profiling) issues them
-   one by one too, so the events around the loading are recorded.  Results
-   do not depend on it. */
-int tapgpu_set_graphs(tapgpu_ctx *ctx, int on);
-int tapgpu_get_graphs(tapgpu_ctx *ctx, int *on);
+   reference, whose steps are host loops).  mode = 1 (the default): once the
+   loading's schedule is tuned, each iteration (opening kernel, loading,
+   both line-search windows, step, readback) is a replay of a captured HIP
+   graph -- on the per-origin LDS and HBM paths and on the lanes kernel;
+   elsewhere the link passes are.  On the per-origin paths an iteration
+   shorter than 0.5 ms goes 8 to a replay (2 below 2 ms), so the launch and
+   the synchronisation are paid once for several.  mode = 2..8: that many
+   iterations per replay on those paths.  mode = 0: the same launches issued
+   one by one.  Profiling (tapgpu_set_profiling) issues them one by one too,
+   so the events around the loading are recorded.  Results do not depend on
+   it. */
+int tapgpu_set_graphs(tapgpu_ctx *ctx, int mode);
+int tapgpu_get_graphs(tapgpu_ctx *ctx, int *mode);
 
 /* --- greedy step-cost assignment ------------------------------------------ */
 

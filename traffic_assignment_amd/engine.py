@@ -402,16 +402,17 @@ class TapEngine:
     def lanes_delta(self):
         return lib().tapgpu_get_lanes_delta(self._h)
 
-    def set_graphs(self, on=True):
-        """Replay each Frank-Wolfe iteration from a captured HIP graph (on,
-        the default) or issue its launches one by one."""
-        self._check(lib().tapgpu_set_graphs(self._h, int(bool(on))))
+    def set_graphs(self, mode=1):
+        """Replay Frank-Wolfe iterations from captured HIP graphs (1, the
+        default: iterations per replay from the iteration time; 2..8: that
+        many) or issue their launches one by one (0 / False)."""
+        self._check(lib().tapgpu_set_graphs(self._h, int(mode)))
 
     @property
     def graphs(self):
         v = C.c_int()
         self._check(lib().tapgpu_get_graphs(self._h, C.byref(v)))
-        return bool(v.value)
+        return v.value
 
     def set_profiling(self, on=True):
         self._check(lib().tapgpu_set_profiling(self._h, int(bool(on))))
