@@ -1303,9 +1303,13 @@ __device__ void block_reduce_acc(tap_acc* v, tap_acc* sh /* [kBlock / kWave][kQ]
 }
 
 // Partial sums per (variant, block): g0 = sum c*d, objective, vht and
-// g(2^-j) for j in [jbase, jbase + kJ) (j = 0 is lambda = 1).
-__global__ __launch_bounds__(kBlock) void k_link_partials(Links K, const VarState* st,
-                                                          tap_acc* part, int only_g) {
+// g(2^-j) for j in [jbase, jbase + kJ) (j = 0 is lambda = 1).  A block is
+// one wavefront (kLinkThreads): each lane sums several links, then one
+// shuffle reduction per accumulator, with no LDS stage.
+constexpr int kLinkThreads = kWave;
+
+__global__ __launch_bounds__(kLinkThreads) void k_link_partials(Links K, const VarState* st,
+                                                                tap_acc* part, int only_g) {
   const int var = blockIdx.y;
   const VarState s = st[var];
   if (!s.stepping || (only_g && !s.need_more)) return;
@@ -1315,7 +1319,7 @@ __global__ __launch_bounds__(kBlock) void k_link_partials(Links K, const VarStat
   double lam[kJ];
   for (int j = 0; j < kJ; ++j) lam[j] = ldexp(1.0, -(jbase + j));
   const size_t off = (size_t)var * K.E;
-  for (int k = blockIdx.x * kBlock + threadIdx.x; k < K.E; k += gridDim.x * kBlock) {
+  for (int k = blockIdx.x * kLinkThreads + threadIdx.x; k < K.E; k += gridDim.x * kLinkThreads) {
     const size_t i = off + k;
     if (!K.present[i]) continue;
     const double x = K.x[i];
@@ -1332,8 +1336,15 @@ __global__ __launch_bounds__(kBlock) void k_link_partials(Links K, const VarStat
       tap_acc_add(&acc[3 + j], tap_bpr(t0, cap, B, pw, q) * d);
     }
   }
-  __shared__ tap_acc sh[kBlock / kWave * kQ];
-  block_reduce_acc(acc, sh);
+#pragma unroll
+  for (int q = 0; q < kQ; ++q) {
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+      tap_acc t;
+      t.s = __shfl_xor(acc[q].s, o);
+      t.c = __shfl_xor(acc[q].c, o);
+      tap_acc_merge(&acc[q], t);
+    }
+  }
   if (threadIdx.x == 0) {
     tap_acc* out = part + ((size_t)var * gridDim.x + blockIdx.x) * kQ;
     for (int q = 0; q < kQ; ++q) out[q] = acc[q];
@@ -3056,10 +3067,10 @@ static int upload_params(tapgpu_ctx* c, int max_iters, double target) {
 // copy of the iteration block (error flag, counters, states, flags) to
 // pinned host memory.
 static int issue_link_passes(tapgpu_ctx* c, const Links& K, dim3 lgrid, bool readback = true) {
-  hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kBlock), 0, c->stream, K, c->st, c->part, 0);
+  hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kLinkThreads), 0, c->stream, K, c->st, c->part, 0);
   hipLaunchKernelGGL(k_link_decide, dim3(c->V), dim3(kBlock), 0, c->stream, c->st, c->part,
                      kLinkBlocks, 0, c->prm);
-  hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kBlock), 0, c->stream, K, c->st, c->part, 1);
+  hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kLinkThreads), 0, c->stream, K, c->st, c->part, 1);
   hipLaunchKernelGGL(k_link_decide, dim3(c->V), dim3(kBlock), 0, c->stream, c->st, c->part,
                      kLinkBlocks, 1, c->prm);
   hipLaunchKernelGGL(k_link_step, lgrid, dim3(kBlock), 0, c->stream, K, c->st);
@@ -3315,7 +3326,7 @@ int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_res
     bool more = need_more();
     if (more) c->stats.link_passes += 1;              // the queued second window was used
     while (more) {
-      hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kBlock), 0, c->stream, K, c->st, c->part, 1);
+      hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kLinkThreads), 0, c->stream, K, c->st, c->part, 1);
       hipLaunchKernelGGL(k_link_decide, dim3(c->V), dim3(kBlock), 0, c->stream, c->st, c->part,
                          kLinkBlocks, 1, c->prm);
       hipLaunchKernelGGL(k_link_step, lgrid, dim3(kBlock), 0, c->stream, K, c->st);
