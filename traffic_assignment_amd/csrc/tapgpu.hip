@@ -916,7 +916,7 @@ __device__ __forceinline__ int ldg_i32(const int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-__global__ __launch_bounds__(kHbmBlock, 4) void k_origin_hbm(OriginArgs A, int load, int items) {
+__global__ __launch_bounds__(kHbmBlock, kHbmPerCu * kHbmBlock / kWave / 4) void k_origin_hbm(OriginArgs A, int load, int items) {
   extern __shared__ unsigned long long sm[];
   const int N = A.N;
   const int W = (N + 63) >> 6;
