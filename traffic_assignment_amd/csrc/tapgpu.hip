@@ -488,8 +488,10 @@ struct OriginArgs {
                                      //     tree links loaded, walk steps, -,
                                      //     wall-clock ticks of relaxation,
                                      //     predecessors, loading
-  // shortest-path-only mode: block x handles start node src_list[x]
+  // block x handles origin src_list[x] (shortest-path-only mode: start
+  // nodes; LDS loadings: origins longest first)
   const int* src_list;
+  int* work;                         // [Z] relaxation rounds per origin, recorded when non-null
   double* out_dist;                  // [(N) * ld] node-major, row v = node v+1
   int* out_pred;
   long long ld;
@@ -706,6 +708,7 @@ __global__ __launch_bounds__(BLOCK) void k_origin_lds(OriginArgs A, int load) {
   }
   const int relax_rounds = r;
   const long long clk1 = wall_clock64();
+  if (A.work && tid == 0 && blockIdx.y == 0) A.work[src] = relax_rounds;
 
   // ---- predecessor links (label, then tail id; see above).  The in-link
   //      offsets are staged in LDS (the out-link offsets are dead); each
@@ -1540,6 +1543,12 @@ struct tapgpu_ctx {
   int strategy = 0;
   int chase = -1;                     // LDS path: chained expansions (-1: tune)
   int lds_block = -1;                 // LDS path: workgroup size (-1: tune)
+  // LDS loadings dispatch origins longest first: the first loading records
+  // each origin's relaxation rounds, later ones launch in that order
+  int* d_lds_work = nullptr;          // [Z] rounds per origin
+  int* d_lds_order = nullptr;         // [Z] origins by decreasing rounds
+  bool lds_recorded = false, lds_order_ready = false;
+  std::vector<int> h_lds_order;       // its host copy (the upload's source)
   bool qcap_auto = true;              // LDS path: queue size picked with the workgroup size
   int split = 0;                      // LDS path: origins per variant run by the HBM kernel
                                       // alongside it (-1: tune, 0: none)
@@ -1941,6 +1950,18 @@ int launch_origin(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on, fl
       if (rc) return rc;
       A.h_dist = c->h_dist; A.h_pred = c->h_pred; A.h_ptail = c->h_ptail;
     }
+    if (zh == 0) {
+      if (c->lds_order_ready) {
+        A.src_list = c->d_lds_order;
+      } else if (!c->capturing && c->Z > 1) {
+        if (!c->d_lds_work) {
+          int rc = dalloc(c, &c->d_lds_work, (size_t)c->Z);
+          if (rc) return rc;
+        }
+        A.work = c->d_lds_work;
+        c->lds_recorded = true;
+      }
+    }
     if (ms) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     if (zh > 0) {
       HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
@@ -1948,6 +1969,8 @@ int launch_origin(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on, fl
       OriginArgs B = A;
       B.z0 = zl;
       B.zn = zh;
+      B.src_list = nullptr;
+      B.work = nullptr;
       const int items = zh * Vn;
       hipLaunchKernelGGL(k_origin_hbm, dim3(std::min(c->hbm_grid, items)), dim3(kHbmBlock),
                          c->hbm_lds, c->stream2, B, 1, items);
@@ -2121,8 +2144,31 @@ int absorb_origin(tapgpu_ctx* c, const int* herr, const unsigned long long* hc, 
   return TAPGPU_OK;
 }
 
+// After a loading that recorded each origin's relaxation rounds: the
+// origins sorted by decreasing rounds (then id), the dispatch order of later
+// LDS loadings.  The hardware starts workgroups in block order as slots
+// free up, so the longest searches start first and the short ones fill the
+// tail.  Results do not depend on the order.
+int lds_order_update(tapgpu_ctx* c) {
+  if (!c->lds_recorded || c->lds_order_ready || c->capturing) return TAPGPU_OK;
+  std::vector<int> work(c->Z), order(c->Z);
+  HIPCHK(c, hipMemcpyAsync(work.data(), c->d_lds_work, sizeof(int) * c->Z, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int z = 0; z < c->Z; ++z) order[z] = z;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+  int rc;
+  if (!c->d_lds_order && (rc = dalloc(c, &c->d_lds_order, (size_t)c->Z))) return rc;
+  if ((rc = upload(c, c->d_lds_order, order.data(), (size_t)c->Z))) return rc;
+  c->h_lds_order = std::move(order);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->lds_order_ready = true;
+  return TAPGPU_OK;
+}
+
 int aon_lds(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
   auto t0 = std::chrono::steady_clock::now();
+  if (int rc = lds_order_update(c)) return rc;
   if (!c->pre_cleared) HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
   if (c->lds_block < 0 && c->Z > 0) {
     // workgroup and queue size first, then the chain length, on the LDS
@@ -3129,7 +3175,8 @@ static int iter_graph(tapgpu_ctx* c, const Links& K, dim3 lgrid, int m) {
   }
   long long dbits;
   std::memcpy(&dbits, &c->delta_mult, sizeof dbits);
-  const long long key[6] = {(lds ? 1 : 2) + 4 * m, c->chase, c->qcap, c->hqcap, dbits, lds_threads(c)};
+  const long long key[6] = {(lds ? 1 : 2) + 4 * m + (c->lds_order_ready ? 64 : 0), c->chase, c->qcap,
+                            c->hqcap, dbits, lds_threads(c)};
   if (c->g_iter && std::equal(key, key + 6, c->g_iter_key)) return TAPGPU_OK;
   int rc = capture(c, &c->g_iter, [&]() -> int {
     for (int it = 0; it < m; ++it) {
@@ -3237,6 +3284,7 @@ int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_res
       }
     }
     auto t_it = std::chrono::steady_clock::now();
+    if (use_lds(c) && (rc = lds_order_update(c))) return rc;    // (before any capture)
     // with the schedule already tuned, one kernel opens the iteration and
     // clears what the loading would clear
     const bool tuned = use_lds(c) ? (c->chase >= 0 && c->split >= 0 && c->lds_block > 0)
