@@ -492,6 +492,11 @@ struct OriginArgs {
   // nodes; LDS loadings: origins longest first)
   const int* src_list;
   int* work;                         // [Z] relaxation rounds per origin, recorded when non-null
+  // labels-in-HBM loadings: items taken from a counter (null: block-strided),
+  // origin-major in `order` (origins by decreasing rounds) when non-null
+  int* next;
+  const int* order;
+  int vn;                            // variants of the launch (with order)
   double* out_dist;                  // [(N) * ld] node-major, row v = node v+1
   int* out_pred;
   long long ld;
@@ -902,6 +907,8 @@ constexpr int kHbmBlock = TAPGPU_HBM_BLOCK;    // 512: 2 workgroups per CU, each
 constexpr int kHbmPerCu = TAPGPU_HBM_PER_CU;    // workgroups sharing a CU's LDS (registers: <= 128 VGPRs)
 constexpr int kHbmQueueMax = 4096;              // frontier queue entries per round
 
+__global__ void k_zero_i32(int* p) { *p = 0; }
+
 // LDS of the labels-in-HBM kernel: three bitmaps 3*8W | two queues of
 // (label 8 B, node 4 B) entries 2*12Q.
 size_t origin_hbm_lds_bytes(int N, int hqcap) {
@@ -948,9 +955,27 @@ __global__ __launch_bounds__(kHbmBlock, kHbmPerCu * kHbmBlock / kWave / 4) void 
   unsigned long long n_def = 0;
   unsigned long long c_relax = 0, c_pred = 0, c_load = 0;
 
-  for (int item = blockIdx.x; item < items; item += gridDim.x) {
-    const int var = A.src_list ? 0 : item / A.zn;
-    const int src = A.src_list ? A.src_list[item] : A.z0 + item - var * A.zn;
+  __shared__ int s_item;
+  for (int k = 0;; ++k) {
+    // the next item: from the launch's counter (the first free workgroup
+    // takes it), or block-strided
+    if (tid == 0) s_item = A.next ? atomicAdd(A.next, 1) : (int)blockIdx.x + k * (int)gridDim.x;
+    __syncthreads();
+    const int item = s_item;
+    __syncthreads();
+    if (item >= items) break;
+    int var, src;
+    if (A.src_list) {
+      var = 0;
+      src = A.src_list[item];
+    } else if (A.order) {                     // longest origins first, all variants of each
+      const int rank = item / A.vn;
+      var = item - rank * A.vn;
+      src = A.order[rank];
+    } else {
+      var = item / A.zn;
+      src = A.z0 + item - var * A.zn;
+    }
     if (A.var_on && !A.var_on[var]) continue;
     const double* __restrict__ fw = A.fww + (size_t)var * A.E;
     const double* __restrict__ rw = A.rvw + (size_t)var * A.E;
@@ -1095,6 +1120,7 @@ __global__ __launch_bounds__(kHbmBlock, kHbmPerCu * kHbmBlock / kWave / 4) void 
       unsigned long long* t = cur; cur = nxt; nxt = clr; clr = t;
     }
     n_rounds += (unsigned long long)r;
+    if (A.work && tid == 0 && var == 0) A.work[src] = r;
     const long long clk1 = wall_clock64();
 
     // ---- predecessor links: smallest tail label below the node's, then
@@ -1545,10 +1571,11 @@ struct tapgpu_ctx {
   int lds_block = -1;                 // LDS path: workgroup size (-1: tune)
   // LDS loadings dispatch origins longest first: the first loading records
   // each origin's relaxation rounds, later ones launch in that order
-  int* d_lds_work = nullptr;          // [Z] rounds per origin
-  int* d_lds_order = nullptr;         // [Z] origins by decreasing rounds
-  bool lds_recorded = false, lds_order_ready = false;
-  std::vector<int> h_lds_order;       // its host copy (the upload's source)
+  int* d_origin_work = nullptr;          // [Z] rounds per origin
+  int* d_hbm_next = nullptr;          // item counter of a labels-in-HBM loading
+  int* d_origin_order = nullptr;         // [Z] origins by decreasing rounds
+  bool origin_recorded = false, origin_order_ready = false;
+  std::vector<int> h_origin_order;       // its host copy (the upload's source)
   bool qcap_auto = true;              // LDS path: queue size picked with the workgroup size
   int split = 0;                      // LDS path: origins per variant run by the HBM kernel
                                       // alongside it (-1: tune, 0: none)
@@ -1640,6 +1667,7 @@ struct tapgpu_ctx {
   std::vector<int> h_lane_var;        // its host copy (as uploaded)
   unsigned l_stamp = 1;               // stamp of the next task's memo entries
   unsigned* l_stamp_dev = nullptr;    // its device copy, read by the kernel
+  int* l_next = nullptr;              // task counter of a lanes launch
   bool capturing = false;             // launches are being recorded into a graph
   // stats
   tapgpu_stats stats{};
@@ -1951,15 +1979,15 @@ int launch_origin(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on, fl
       A.h_dist = c->h_dist; A.h_pred = c->h_pred; A.h_ptail = c->h_ptail;
     }
     if (zh == 0) {
-      if (c->lds_order_ready) {
-        A.src_list = c->d_lds_order;
+      if (c->origin_order_ready) {
+        A.src_list = c->d_origin_order;
       } else if (!c->capturing && c->Z > 1) {
-        if (!c->d_lds_work) {
-          int rc = dalloc(c, &c->d_lds_work, (size_t)c->Z);
+        if (!c->d_origin_work) {
+          int rc = dalloc(c, &c->d_origin_work, (size_t)c->Z);
           if (rc) return rc;
         }
-        A.work = c->d_lds_work;
-        c->lds_recorded = true;
+        A.work = c->d_origin_work;
+        c->origin_recorded = true;
       }
     }
     if (ms) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
@@ -2149,26 +2177,26 @@ int absorb_origin(tapgpu_ctx* c, const int* herr, const unsigned long long* hc, 
 // LDS loadings.  The hardware starts workgroups in block order as slots
 // free up, so the longest searches start first and the short ones fill the
 // tail.  Results do not depend on the order.
-int lds_order_update(tapgpu_ctx* c) {
-  if (!c->lds_recorded || c->lds_order_ready || c->capturing) return TAPGPU_OK;
+int origin_order_update(tapgpu_ctx* c) {
+  if (!c->origin_recorded || c->origin_order_ready || c->capturing) return TAPGPU_OK;
   std::vector<int> work(c->Z), order(c->Z);
-  HIPCHK(c, hipMemcpyAsync(work.data(), c->d_lds_work, sizeof(int) * c->Z, hipMemcpyDeviceToHost,
+  HIPCHK(c, hipMemcpyAsync(work.data(), c->d_origin_work, sizeof(int) * c->Z, hipMemcpyDeviceToHost,
                            c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (int z = 0; z < c->Z; ++z) order[z] = z;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
   int rc;
-  if (!c->d_lds_order && (rc = dalloc(c, &c->d_lds_order, (size_t)c->Z))) return rc;
-  if ((rc = upload(c, c->d_lds_order, order.data(), (size_t)c->Z))) return rc;
-  c->h_lds_order = std::move(order);
+  if (!c->d_origin_order && (rc = dalloc(c, &c->d_origin_order, (size_t)c->Z))) return rc;
+  if ((rc = upload(c, c->d_origin_order, order.data(), (size_t)c->Z))) return rc;
+  c->h_origin_order = std::move(order);
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  c->lds_order_ready = true;
+  c->origin_order_ready = true;
   return TAPGPU_OK;
 }
 
 int aon_lds(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
   auto t0 = std::chrono::steady_clock::now();
-  if (int rc = lds_order_update(c)) return rc;
+  if (int rc = origin_order_update(c)) return rc;
   if (!c->pre_cleared) HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
   if (c->lds_block < 0 && c->Z > 0) {
     // workgroup and queue size first, then the chain length, on the LDS
@@ -2210,6 +2238,7 @@ int ensure_hbm(tapgpu_ctx* c) {
   if (c->h_dist) return TAPGPU_OK;
   const size_t n = (size_t)c->hbm_grid * c->N;
   int rc;
+  if ((rc = dalloc(c, &c->d_hbm_next, 1))) return rc;
   if ((rc = dalloc(c, &c->h_dist, n))) return rc;
   if ((rc = dalloc(c, &c->h_pred, n))) return rc;
   if ((rc = dalloc(c, &c->h_ptail, n))) return rc;
@@ -2235,6 +2264,20 @@ int launch_hbm(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on, int i
   if (var_on) A.var_on = var_on + v0;
   const int items = std::min(c->Z * Vn, items_cap);
   const int G = std::min(c->hbm_grid, items);
+  if (int rc = ensure_hbm(c)) return rc;        // (the item counter; no allocation once made)
+  if (c->origin_order_ready) {
+    A.order = c->d_origin_order;
+    A.vn = Vn;
+  } else if (!c->capturing && c->Z > 1) {
+    if (!c->d_origin_work) {
+      int rc = dalloc(c, &c->d_origin_work, (size_t)c->Z);
+      if (rc) return rc;
+    }
+    A.work = c->d_origin_work;
+    c->origin_recorded = true;
+  }
+  A.next = c->d_hbm_next;
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, c->stream, c->d_hbm_next);
   if (ms) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
   hipLaunchKernelGGL(k_origin_hbm, dim3(G), dim3(kHbmBlock), c->hbm_lds, c->stream, A, 1, items);
   HIPCHK(c, hipGetLastError());
@@ -2272,6 +2315,7 @@ int aon_hbm(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
   auto t0 = std::chrono::steady_clock::now();
   int rc = ensure_hbm(c);
   if (rc) return rc;
+  if ((rc = origin_order_update(c))) return rc;
   if (!c->pre_cleared) HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
   if (c->delta_mult < 0 && c->Z > 0 && (rc = tune_delta(c, v0, Vn, var_on))) return rc;
   if (!c->pre_cleared)
@@ -3175,7 +3219,7 @@ static int iter_graph(tapgpu_ctx* c, const Links& K, dim3 lgrid, int m) {
   }
   long long dbits;
   std::memcpy(&dbits, &c->delta_mult, sizeof dbits);
-  const long long key[6] = {(lds ? 1 : 2) + 4 * m + (c->lds_order_ready ? 64 : 0), c->chase, c->qcap,
+  const long long key[6] = {(lds ? 1 : 2) + 4 * m + (c->origin_order_ready ? 64 : 0), c->chase, c->qcap,
                             c->hqcap, dbits, lds_threads(c)};
   if (c->g_iter && std::equal(key, key + 6, c->g_iter_key)) return TAPGPU_OK;
   int rc = capture(c, &c->g_iter, [&]() -> int {
@@ -3207,7 +3251,8 @@ static int iter_graph(tapgpu_ctx* c, const Links& K, dim3 lgrid, int m) {
 static int lanes_graph(tapgpu_ctx* c, const Links& K, dim3 lgrid, int G) {
   long long dbits;
   std::memcpy(&dbits, &c->lanes_delta_mult, sizeof dbits);
-  const long long key[6] = {3, G, c->lanes_qcap, dbits, (long long)c->lanes_lds, c->lanes_grid};
+  const long long key[6] = {3 + (c->origin_order_ready ? 64 : 0), G, c->lanes_qcap, dbits,
+                            (long long)c->lanes_lds, c->lanes_grid};
   if (c->g_iter && std::equal(key, key + 6, c->g_iter_key)) return TAPGPU_OK;
   c->capturing = true;
   int rc = capture(c, &c->g_iter, [&]() {
@@ -3284,7 +3329,7 @@ int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_res
       }
     }
     auto t_it = std::chrono::steady_clock::now();
-    if (use_lds(c) && (rc = lds_order_update(c))) return rc;    // (before any capture)
+    if ((rc = origin_order_update(c))) return rc;      // (before any capture)
     // with the schedule already tuned, one kernel opens the iteration and
     // clears what the loading would clear
     const bool tuned = use_lds(c) ? (c->chase >= 0 && c->split >= 0 && c->lds_block > 0)
