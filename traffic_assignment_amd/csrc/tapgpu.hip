@@ -1958,6 +1958,15 @@ constexpr int kChaseCandidates[] = {0, 2, 4, 8, 16, 32};
 
 int ensure_hbm(tapgpu_ctx* c);
 
+// Per-origin round counts of the recording loading (zeroed: origins a
+// loading skips keep 0 and go last).
+int ensure_origin_work(tapgpu_ctx* c) {
+  int rc = dalloc(c, &c->d_origin_work, (size_t)c->Z);
+  if (rc) return rc;
+  HIPCHK(c, hipMemsetAsync(c->d_origin_work, 0, sizeof(int) * (size_t)c->Z, c->stream));
+  return TAPGPU_OK;
+}
+
 // One loading launch of the LDS path.  With split > 0 the last `split`
 // origins of every variant go to the labels-in-HBM kernel on a second queue,
 // running on the same CUs at the same time: the LDS kernel's workgroup takes
@@ -1983,7 +1992,7 @@ int launch_origin(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on, fl
         A.src_list = c->d_origin_order;
       } else if (!c->capturing && c->Z > 1) {
         if (!c->d_origin_work) {
-          int rc = dalloc(c, &c->d_origin_work, (size_t)c->Z);
+          int rc = ensure_origin_work(c);
           if (rc) return rc;
         }
         A.work = c->d_origin_work;
@@ -2270,7 +2279,7 @@ int launch_hbm(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on, int i
     A.vn = Vn;
   } else if (!c->capturing && c->Z > 1) {
     if (!c->d_origin_work) {
-      int rc = dalloc(c, &c->d_origin_work, (size_t)c->Z);
+      int rc = ensure_origin_work(c);
       if (rc) return rc;
     }
     A.work = c->d_origin_work;
