@@ -204,10 +204,11 @@ def _acc(acc, st):
         acc[k] = acc.get(k, 0) + v
 
 
-def solve_batch(eng, target, tag, chunk=25):
-    """fw_init, then chunks of FW iterations until every variant has
-    converged (a progress line per chunk); the per-variant results."""
-    eng.fw_init()
+def solve_batch(eng, target, tag, chunk=25, warm=None):
+    """fw_init (from warm volumes when given), then chunks of FW iterations
+    until every variant has converged (a progress line per chunk); the
+    per-variant results."""
+    eng.fw_init(warm)
     done = 0
     while True:
         done += chunk
@@ -244,6 +245,57 @@ def regional_sssp(args):
 def eng_can_block(eng, block):
     """A fixed LDS workgroup size applies to the LDS path only."""
     return block > 0 and eng.strategy == "lds"
+
+
+def warm_started(args, eng, base, dem, mods, subsets, u, present):
+    """The batch again, every variant warm-started from the base network's
+    equilibrium (a feasible flow of every variant: variants only change link
+    records or add links, which start empty), against the same plain solves
+    warm-started the same way.  The use the reference's warm start (-w
+    flows file) serves: fewer FW iterations per upgrade variant."""
+    import torch
+    from traffic_assignment_amd import variants as V
+    from traffic_assignment_amd.engine import TapEngine
+
+    t = time.perf_counter()
+    with TapEngine(base, dem) as be:
+        be.fw_init()
+        be.fw_run(target_relgap=TARGET)
+        xb, _ = be.flows()
+    base_s = time.perf_counter() - t
+    index = {(int(a), int(b)): k for k, (a, b) in enumerate(zip(u.init, u.term))}
+    xu = np.zeros(u.num_links)
+    xu[[index[(int(a), int(b))] for a, b in zip(base.init, base.term)]] = xb
+    Vn = len(subsets)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    res = solve_batch(eng, TARGET, "warm batch solve", warm=np.tile(xu, (Vn, 1)))
+    torch.cuda.synchronize()
+    batch_s = time.perf_counter() - t1
+    assert all(r["converged"] for r in res), "a warm-started variant did not converge"
+    iters = [r["iterations"] for r in res]
+    sample = list(range(0, Vn, max(1, Vn // args.plain_sample)))[:args.plain_sample]
+    plain_t = 0.0
+    for v in sample:
+        net = V.variant_network(base, mods, subsets[v])
+        xv = xu[present[v].astype(bool)]
+        with TapEngine(net, dem) as pe:
+            pe.fw_init()            # untimed: schedule tuning and first-launch costs
+            pe.fw_run(max_iters=1)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            pe.fw_init(xv)
+            r = pe.fw_run(target_relgap=TARGET)[0]
+            torch.cuda.synchronize()
+            plain_t += time.perf_counter() - t
+            assert r["iterations"] == iters[v], "warm plain and batched iteration counts differ"
+    plain = len(sample) / plain_t
+    batched = Vn / batch_s
+    print(f"warm start: batched {batched:.3f}, plain {plain:.3f} solves/s", file=sys.stderr, flush=True)
+    return {"batched_solves_per_s": round(batched, 5), "plain_solves_per_s": round(plain, 5),
+            "batched_vs_plain": round(batched / plain, 3), "batch_solve_s": round(batch_s, 3),
+            "base_solve_s": round(base_s, 3), "plain_sample": len(sample),
+            "iterations_min_median_max": [min(iters), int(np.median(iters)), max(iters)]}
 
 
 def bench_batched(args):
@@ -294,10 +346,11 @@ def bench_batched(args):
     res = solve_batch(eng, TARGET, "batch solve")
     torch.cuda.synchronize()
     batch_s = time.perf_counter() - t1
-    eng.close()
     iters = [r["iterations"] for r in res]
     assert all(r["converged"] for r in res), "a variant did not converge"
     batched = len(subsets) / batch_s
+    warm = None if args.no_warm else warm_started(args, eng, base, dem, mods, subsets, u, present)
+    eng.close()
     # plain: single-variant engines, one after another (same GPU, default path)
     sample = list(range(0, len(subsets), max(1, len(subsets) // args.plain_sample)))[:args.plain_sample]
     plain_t, plain_it = 0.0, []
@@ -344,6 +397,7 @@ def bench_batched(args):
         "roofline": roof,
         "cpu_baseline": cpu,
         "sssp_regional": sssp,
+        "warm_start": warm,
     }
     if sssp:
         out["sssp_gbps"] = sssp["achieved"]
@@ -444,6 +498,8 @@ def main():
                     help="batched: lanes kernel bucket width in mean free-flow times (-1: tuned)")
     ap.add_argument("--no-solve", action="store_true",
                     help="batched: time the K steps only (profiling runs)")
+    ap.add_argument("--no-warm", action="store_true",
+                    help="batched: skip the warm-started batch and plain solves")
     ap.add_argument("--no-regional", action="store_true",
                     help="batched: skip the regional shortest-path measurement")
     ap.add_argument("--regional-iters", type=int, default=2,

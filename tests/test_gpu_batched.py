@@ -88,3 +88,53 @@ def test_batched_variants_stop_independently(engine_mod):
         x_resumed = [eng2.flows(v)[0] for v in range(len(subsets))]
     assert [r["iterations"] for r in done] == iters
     assert all(np.array_equal(a, b) for a, b in zip(x_before, x_resumed))
+
+
+@pytest.mark.parametrize("strategy", ["lds", "lanes"])
+def test_batched_warm_start_from_base_equilibrium(engine_mod, oracle_lib, strategy):
+    """Every variant warm-started from the base network's equilibrium (added
+    links start empty): each variant's λ sequence and volumes equal its plain
+    solve from the same volumes and the oracle's, and the warm start needs
+    fewer iterations than the cold one."""
+    print(f"case batched warm start {strategy}", flush=True)
+    base, dem, mods, subsets = _small()
+    with engine_mod.TapEngine(base, dem) as be:
+        be.fw_init()
+        be.fw_run(target_relgap=1e-4)
+        xb, _ = be.flows()
+    eng, u, present = variants.make_engine(base, dem, mods, subsets)
+    index = {(int(a), int(b)): k for k, (a, b) in enumerate(zip(u.init, u.term))}
+    xu = np.zeros(u.num_links)
+    xu[[index[(int(a), int(b))] for a, b in zip(base.init, base.term)]] = xb
+    with eng:
+        if strategy == "lanes":
+            eng.set_lanes(4)
+        else:
+            eng.set_strategy(strategy)
+            eng.set_lanes(0)
+        eng.fw_init()
+        cold = eng.fw_run(target_relgap=1e-4)
+        eng.fw_init(np.tile(xu, (len(subsets), 1)))
+        res = eng.fw_run(target_relgap=1e-4)
+        out = [(eng.flows(v)[0], eng.trace(v)["lambda_"]) for v in range(len(subsets))]
+    assert all(r["converged"] for r in res)
+    assert sum(r["iterations"] for r in res) < sum(r["iterations"] for r in cold)
+    for v in (0, 5, 11):
+        net = variants.variant_network(base, mods, subsets[v])
+        m = present[v].astype(bool)
+        with engine_mod.TapEngine(net, dem) as pe:
+            pe.fw_init(xu[m])
+            pr = pe.fw_run(target_relgap=1e-4)[0]
+            px, _ = pe.flows()
+            plam = pe.trace()["lambda_"]
+        bx, blam = out[v]
+        assert pr["iterations"] == res[v]["iterations"], v
+        assert np.array_equal(blam, plam) and np.array_equal(bx[m], px), v
+        if v == 0:
+            oracle_lib.lib().ora_set_pow_mode(1)
+            try:
+                o = oracle_lib.fw_solve(net, dem, target_relgap=1e-4, nthreads=8, x_init=xu[m])
+            finally:
+                oracle_lib.lib().ora_set_pow_mode(0)
+            assert o["iterations"] == pr["iterations"]
+            assert np.array_equal(o["trace"]["lambda_"], plam) and np.array_equal(o["x"], px)
