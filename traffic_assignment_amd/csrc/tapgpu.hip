@@ -1383,24 +1383,15 @@ __global__ __launch_bounds__(kLinkThreads) void k_link_partials(Links K, const V
 // Per variant: combine partials, pick lambda (first 2^-j with g <= 0, the
 // reference's rule), then gap, objective, lower bound, relative gap, average
 // excess cost, trace record, stop test.
-__global__ __launch_bounds__(kBlock) void k_link_decide(VarState* st, const tap_acc* part,
-                                                        int nparts, int only_g,
-                                                        const FwParams* prm) {
+// The decision of one window for one variant from its merged sums acc[kQ]
+// (thread 0 of the caller): on the first window the iteration's scalars
+// and trace record, then lambda = the first 2^-j with g <= 0, or a request
+// for the next window (need_more, jnext).
+__device__ void decide_window(VarState& s, const tap_acc* acc, int only_g, const FwParams* prm,
+                              int var) {
   tapgpu_iter* const trace = prm->trace;
   const int trace_cap = prm->trace_cap, max_iters = prm->max_iters;
   const double target = prm->target;
-  const int var = blockIdx.x;
-  VarState s = st[var];
-  if (!s.stepping || (only_g && !s.need_more)) return;
-  tap_acc acc[kQ];
-  for (int q = 0; q < kQ; ++q) acc[q] = tap_acc{0.0, 0.0};
-  for (int b = threadIdx.x; b < nparts; b += kBlock) {
-    const tap_acc* in = part + ((size_t)var * nparts + b) * kQ;
-    for (int q = 0; q < kQ; ++q) tap_acc_merge(&acc[q], in[q]);
-  }
-  __shared__ tap_acc sh[kBlock / kWave * kQ];
-  block_reduce_acc(acc, sh);
-  if (threadIdx.x != 0) return;
   double gj[kJ];
   for (int j = 0; j < kJ; ++j) gj[j] = tap_acc_value(acc[3 + j]);
   const int jbase = only_g ? s.jnext : 0;
@@ -1452,6 +1443,24 @@ __global__ __launch_bounds__(kBlock) void k_link_decide(VarState* st, const tap_
     s.iter += 1;
     s.active = (max_iters <= 0 || s.iter < max_iters) && (target <= 0 || s.relgap > target);
   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_link_decide(VarState* st, const tap_acc* part,
+                                                        int nparts, int only_g,
+                                                        const FwParams* prm) {
+  const int var = blockIdx.x;
+  VarState s = st[var];
+  if (!s.stepping || (only_g && !s.need_more)) return;
+  tap_acc acc[kQ];
+  for (int q = 0; q < kQ; ++q) acc[q] = tap_acc{0.0, 0.0};
+  for (int b = threadIdx.x; b < nparts; b += kBlock) {
+    const tap_acc* in = part + ((size_t)var * nparts + b) * kQ;
+    for (int q = 0; q < kQ; ++q) tap_acc_merge(&acc[q], in[q]);
+  }
+  __shared__ tap_acc sh[kBlock / kWave * kQ];
+  block_reduce_acc(acc, sh);
+  if (threadIdx.x != 0) return;
+  decide_window(s, acc, only_g, prm, var);
   st[var] = s;
 }
 
@@ -1474,9 +1483,77 @@ __global__ __launch_bounds__(kBlock) void k_link_step(Links K, VarState* st) {
   }
 }
 
-// Total link cost sum_k x_k c_k of every variant (one block each): the
-// compensated sums of the threads' strided links, merged in a fixed order,
-// so the result does not depend on timing.  Absent links carry x = 0.
+// All link passes of an iteration for a small network, one workgroup per
+// variant and one launch: the window sums (block reduction), the decision,
+// further windows while the step is undecided, the step (x, costs, both
+// cost mirrors) and the flags.  A workgroup owns its variant's links, so
+// nothing waits on other workgroups.  Used where links <= kSmallLinks;
+// the sums are the same compensated sums as the multi-block passes.
+constexpr int kSmallLinks = 4096;
+
+__global__ __launch_bounds__(kBlock) void k_link_small(Links K, VarState* st, unsigned char* var_on,
+                                                       const FwParams* prm) {
+  const int var = blockIdx.x;
+  VarState s = st[var];
+  if (!s.stepping) return;
+  __shared__ tap_acc sh[kBlock / kWave * kQ];
+  __shared__ int s_more, s_jnext;
+  __shared__ double s_lambda;
+  const size_t off = (size_t)var * K.E;
+  int jbase = 0;                     // first window, then the ones thread 0's decisions ask for
+  for (int only_g = 0;; only_g = 1) {
+    tap_acc acc[kQ];
+    for (int q = 0; q < kQ; ++q) acc[q] = tap_acc{0.0, 0.0};
+    double lam[kJ];
+    for (int j = 0; j < kJ; ++j) lam[j] = ldexp(1.0, -(jbase + j));
+    for (int k = threadIdx.x; k < K.E; k += kBlock) {
+      const size_t i = off + k;
+      if (!K.present[i]) continue;
+      const double x = K.x[i];
+      const double d = K.y[i] - x;
+      const double t0 = K.t0[i], cap = K.cap[i], B = K.B[i], pw = K.pw[i];
+      if (!only_g) {
+        const double c = K.c[i];
+        tap_acc_add(&acc[0], c * d);
+        tap_acc_add(&acc[1], tap_integral(t0, cap, B, pw, x));
+        tap_acc_add(&acc[2], x * c);
+      }
+      for (int j = 0; j < kJ; ++j) {
+        const double q = x + lam[j] * d;
+        tap_acc_add(&acc[3 + j], tap_bpr(t0, cap, B, pw, q) * d);
+      }
+    }
+    block_reduce_acc(acc, sh);
+    if (threadIdx.x == 0) {
+      decide_window(s, acc, only_g, prm, var);
+      s_more = s.need_more;
+      s_jnext = s.jnext;
+      s_lambda = s.lambda;
+    }
+    __syncthreads();
+    if (!s_more) break;
+    jbase = s_jnext;
+    __syncthreads();                   // (s_more and s_jnext are rewritten next window)
+  }
+  const double lambda = s_lambda;
+  for (int k = threadIdx.x; k < K.E; k += kBlock) {
+    const size_t i = off + k;
+    const double d = K.y[i] - K.x[i];
+    const double x = K.x[i] + lambda * d;
+    K.x[i] = x;
+    const double c = tap_bpr(K.t0[i], K.cap[i], K.B[i], K.pw[i], x);
+    K.c[i] = c;
+    const double wv = K.present[i] ? c : INFINITY;
+    K.rvw[off + K.rvpos[k]] = wv;
+    K.fww[i] = wv;
+  }
+  if (threadIdx.x == 0) {
+    s.stepping = 0;
+    st[var] = s;
+    var_on[var] = (unsigned char)s.active;
+  }
+}
+
 // Per variant at the current volumes: out[2v] = VHT = sum x * c, out[2v+1]
 // = the Beckmann objective (sum of the link cost integrals), compensated
 // sums in a fixed order.
@@ -3166,6 +3243,15 @@ static int upload_params(tapgpu_ctx* c, int max_iters, double target) {
 // copy of the iteration block (error flag, counters, states, flags) to
 // pinned host memory.
 static int issue_link_passes(tapgpu_ctx* c, const Links& K, dim3 lgrid, bool readback = true) {
+  if (c->E <= kSmallLinks) {          // one launch, one workgroup per variant
+    hipLaunchKernelGGL(k_link_small, dim3(c->V), dim3(kBlock), 0, c->stream, K, c->st, c->var_on,
+                       c->prm);
+    HIPCHK(c, hipGetLastError());
+    if (readback)
+      HIPCHK(c, hipMemcpyAsync(c->pin_block, c->iter_dev, c->iter_bytes, hipMemcpyDeviceToHost,
+                               c->stream));
+    return TAPGPU_OK;
+  }
   hipLaunchKernelGGL(k_link_partials, lgrid, dim3(kLinkThreads), 0, c->stream, K, c->st, c->part, 0);
   hipLaunchKernelGGL(k_link_decide, dim3(c->V), dim3(kBlock), 0, c->stream, c->st, c->part,
                      kLinkBlocks, 0, c->prm);
