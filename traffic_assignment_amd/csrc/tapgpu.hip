@@ -3402,12 +3402,13 @@ int tapgpu_fw_run(tapgpu_ctx* c, int max_iters, double target_relgap, tapgpu_res
   }
   if ((rc = upload(c, c->st, hs.data(), c->V))) return rc;
   if ((rc = upload(c, c->var_on, hon.data(), c->V))) return rc;
-  // One synchronisation per iteration: the loading's error flag and
-  // counters, the variants' states and flags are read back together at the
-  // end.  A second line-search window pass is always queued and does
-  // nothing for variants whose lambda the first window decided; a variant
-  // that needs more windows skips the step and is finished by the host loop
-  // below (rare: lambda below 2^-15).
+  // One synchronisation per iteration (or per replay of several): the
+  // loading's error flag and counters, the variants' states and flags are
+  // read back together at the end.  On networks above kSmallLinks links a
+  // second line-search window pass is always queued and does nothing for
+  // variants whose lambda the first window decided; a variant that needs
+  // more windows skips the step and is finished by the host loop below
+  // (rare: lambda below 2^-31).  k_link_small runs every window itself.
   if (!c->ev[2])
     for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
   if ((rc = upload_params(c, max_iters, target_relgap))) return rc;
