@@ -344,6 +344,17 @@ double tapgpu_get_lanes_delta(const tapgpu_ctx *ctx);
 int tapgpu_set_graphs(tapgpu_ctx *ctx, int mode);
 int tapgpu_get_graphs(tapgpu_ctx *ctx, int *mode);
 
+/* Form of the link passes (the line search and the step; no counterpart in
+   the reference).  0 (the default): networks of up to 4,096 links run all of
+   an iteration's link work in one launch, one workgroup per variant (every
+   line-search window included); larger ones run multi-block passes (two
+   windows queued per iteration, further ones by a host loop).  1 / 2: always
+   the first / the second form.  window: step sizes 2^-j a window decides on
+   (1..16; 0: 16, the default) -- smaller windows exercise the later-window
+   paths on ordinary steps.  The step chosen never depends on either (the
+   first 2^-j with g <= 0); the sums only change order. */
+int tapgpu_set_link_passes(tapgpu_ctx *ctx, int mode, int window);
+
 /* --- greedy step-cost assignment ------------------------------------------ */
 
 typedef struct {

@@ -275,20 +275,26 @@ def test_fw_against_reference_golden(engine_mod, oracle_lib, name, scale, fix, s
     assert np.array_equal(x, o["x"]) and np.array_equal(c, o["c"])
 
 
-def test_fw_line_search_beyond_first_window(engine_mod, oracle_lib):
+@pytest.mark.parametrize("links,window", [(1, 0), (2, 0), (1, 2), (2, 2)])
+def test_fw_line_search_beyond_first_window(engine_mod, oracle_lib, links, window):
     """Past iteration ~7,600 on grid5 the line search needs steps below
-    2^-15, beyond the first window of 16 halvings: the second window is
-    queued with every iteration, later ones are run by the host.  8,450
+    2^-15, beyond the first window of 16 halvings: in the one-launch link
+    kernel (links=1) every window runs in the kernel; in the multi-block
+    passes (links=2) the second window is queued with every iteration and
+    later ones are run by the host.  With windows of 2 step sizes
+    (window=2) nearly every iteration needs later windows, on the host loop
+    too, inside multi-iteration graph replays.  8,450
     iterations (ten such steps) bit-identical to the oracle; in a batched
     engine whose second variant has other capacities, the variants need the
     extra windows at different iterations and each still equals its plain
     solve."""
     import dataclasses
     from traffic_assignment_amd import synth
-    print("case fw deep line search grid5", flush=True)
+    print(f"case fw deep line search grid5, link passes {links}", flush=True)
     it = 8450
     net, dem = network("grid5")
     with engine_mod.TapEngine(net, dem) as eng:
+        eng.set_link_passes(links, window)
         eng.fw_init()
         r = eng.fw_run(max_iters=it)[0]
         x, c = eng.flows()
@@ -308,6 +314,7 @@ def test_fw_line_search_beyond_first_window(engine_mod, oracle_lib):
     rec = {k: np.stack([getattr(net, k)] * 2) for k in ("fftime", "B", "power")}
     with engine_mod.TapEngine(net, dem, num_variants=2, present=np.ones((2, E), np.uint8),
                               capacity=cap2, **rec) as be:
+        be.set_link_passes(links, window)
         be.fw_init()
         res = be.fw_run(max_iters=it)
         bx = [be.flows(v)[0] for v in range(2)]
@@ -316,6 +323,7 @@ def test_fw_line_search_beyond_first_window(engine_mod, oracle_lib):
     assert np.array_equal(blam[0], tr["lambda_"]) and np.array_equal(bx[0], x)
     net2 = dataclasses.replace(net, capacity=0.8 * net.capacity)
     with engine_mod.TapEngine(net2, dem) as pe:
+        pe.set_link_passes(links, window)
         pe.fw_init()
         pe.fw_run(max_iters=it)
         assert np.array_equal(blam[1], pe.trace()["lambda_"])

@@ -1283,6 +1283,7 @@ struct FwParams {
   int trace_cap;
   int max_iters;
   double target;
+  int jwin;                 // step sizes a line-search window decides on (<= kJ)
 };
 
 // mode 0: c = t0 (free-flow, initial Wa); mode 1: c = BPR(x); mode 2: c given
@@ -1392,6 +1393,7 @@ __device__ void decide_window(VarState& s, const tap_acc* acc, int only_g, const
   tapgpu_iter* const trace = prm->trace;
   const int trace_cap = prm->trace_cap, max_iters = prm->max_iters;
   const double target = prm->target;
+  const int jwin = prm->jwin;
   double gj[kJ];
   for (int j = 0; j < kJ; ++j) gj[j] = tap_acc_value(acc[3 + j]);
   const int jbase = only_g ? s.jnext : 0;
@@ -1420,7 +1422,7 @@ __device__ void decide_window(VarState& s, const tap_acc* acc, int only_g, const
     }
   }
   if (!s.found) {
-    for (int j = std::max(jbase, 1); j < jbase + kJ && j <= kMaxHalvings; ++j) {
+    for (int j = std::max(jbase, 1); j < jbase + jwin && j <= kMaxHalvings; ++j) {
       if (gj[j - jbase] <= 0) {
         s.lambda = ldexp(1.0, -j);
         s.found = 1;
@@ -1428,12 +1430,12 @@ __device__ void decide_window(VarState& s, const tap_acc* acc, int only_g, const
       }
     }
     if (!s.found) {
-      if (jbase + kJ > kMaxHalvings) {
+      if (jbase + jwin > kMaxHalvings) {
         s.lambda = 0.0;            // bisection cap reached: lambda_left == 0
         s.found = 1;
       } else {
         s.need_more = 1;
-        s.jnext = jbase + kJ;
+        s.jnext = jbase + jwin;
       }
     }
   }
@@ -1700,6 +1702,9 @@ struct tapgpu_ctx {
   // g_iter = iteration begin + per-origin loading + link passes + readback,
   // g_link = link passes + readback (for the lanes path)
   bool graphs = true;
+  int link_mode = 0;                  // link passes: 0 by size, 1 one workgroup per variant, 2 multi-block
+  int line_window = kJ;               // step sizes per line-search window
+  int g_link_mode = 0;                // the mode g_link was captured with
   int graph_iters = 0;                // iterations per whole-iteration replay (0: from the iteration time)
   double last_iter_ms = 0;            // wall time of the last iteration (per iteration of a replay)
   hipGraphExec_t g_iter = nullptr, g_link = nullptr;
@@ -3233,7 +3238,7 @@ __global__ void k_set_params(FwParams* p, FwParams v) { *p = v; }
 // overwrites)
 static int upload_params(tapgpu_ctx* c, int max_iters, double target) {
   hipLaunchKernelGGL(k_set_params, dim3(1), dim3(1), 0, c->stream, c->prm,
-                     FwParams{c->trace, c->trace_cap, max_iters, target});
+                     FwParams{c->trace, c->trace_cap, max_iters, target, c->line_window});
   HIPCHK(c, hipGetLastError());
   return TAPGPU_OK;
 }
@@ -3243,7 +3248,8 @@ static int upload_params(tapgpu_ctx* c, int max_iters, double target) {
 // copy of the iteration block (error flag, counters, states, flags) to
 // pinned host memory.
 static int issue_link_passes(tapgpu_ctx* c, const Links& K, dim3 lgrid, bool readback = true) {
-  if (c->E <= kSmallLinks) {          // one launch, one workgroup per variant
+  const bool small = c->link_mode == 1 || (c->link_mode == 0 && c->E <= kSmallLinks);
+  if (small) {                        // one launch, one workgroup per variant
     hipLaunchKernelGGL(k_link_small, dim3(c->V), dim3(kBlock), 0, c->stream, K, c->st, c->var_on,
                        c->prm);
     HIPCHK(c, hipGetLastError());
@@ -3314,8 +3320,8 @@ static int iter_graph(tapgpu_ctx* c, const Links& K, dim3 lgrid, int m) {
   }
   long long dbits;
   std::memcpy(&dbits, &c->delta_mult, sizeof dbits);
-  const long long key[6] = {(lds ? 1 : 2) + 4 * m + (c->origin_order_ready ? 64 : 0), c->chase, c->qcap,
-                            c->hqcap, dbits, lds_threads(c)};
+  const long long key[6] = {(lds ? 1 : 2) + 4 * m + (c->origin_order_ready ? 64 : 0) + 128 * c->link_mode,
+                            c->chase, c->qcap, c->hqcap, dbits, lds_threads(c)};
   if (c->g_iter && std::equal(key, key + 6, c->g_iter_key)) return TAPGPU_OK;
   int rc = capture(c, &c->g_iter, [&]() -> int {
     for (int it = 0; it < m; ++it) {
@@ -3346,8 +3352,8 @@ static int iter_graph(tapgpu_ctx* c, const Links& K, dim3 lgrid, int m) {
 static int lanes_graph(tapgpu_ctx* c, const Links& K, dim3 lgrid, int G) {
   long long dbits;
   std::memcpy(&dbits, &c->lanes_delta_mult, sizeof dbits);
-  const long long key[6] = {3 + (c->origin_order_ready ? 64 : 0), G, c->lanes_qcap, dbits,
-                            (long long)c->lanes_lds, c->lanes_grid};
+  const long long key[6] = {3 + (c->origin_order_ready ? 64 : 0) + 128 * c->link_mode, G, c->lanes_qcap,
+                            dbits, (long long)c->lanes_lds, c->lanes_grid};
   if (c->g_iter && std::equal(key, key + 6, c->g_iter_key)) return TAPGPU_OK;
   c->capturing = true;
   int rc = capture(c, &c->g_iter, [&]() {
@@ -3365,8 +3371,16 @@ static int lanes_graph(tapgpu_ctx* c, const Links& K, dim3 lgrid, int G) {
 }
 
 static int link_graph(tapgpu_ctx* c, const Links& K, dim3 lgrid) {
-  if (c->g_link) return TAPGPU_OK;
+  if (c->g_link && c->g_link_mode == c->link_mode) return TAPGPU_OK;
+  c->g_link_mode = c->link_mode;
   return capture(c, &c->g_link, [&]() { return issue_link_passes(c, K, lgrid); });
+}
+
+int tapgpu_set_link_passes(tapgpu_ctx* c, int mode, int window) {
+  if (!c || mode < 0 || mode > 2 || window < 0 || window > kJ) return TAPGPU_E_ARG;
+  c->link_mode = mode;
+  c->line_window = window > 0 ? window : kJ;
+  return TAPGPU_OK;
 }
 
 int tapgpu_set_graphs(tapgpu_ctx* c, int mode) {

@@ -136,6 +136,7 @@ def lib():
         L.tapgpu_get_totals.argtypes = [vp, C.c_int64, vp, vp]
         L.tapgpu_set_links.argtypes = [vp, C.c_int64, vp, vp]
         L.tapgpu_set_graphs.argtypes = [vp, C.c_int]
+        L.tapgpu_set_link_passes.argtypes = [vp, C.c_int, C.c_int]
         L.tapgpu_get_graphs.argtypes = [vp, C.POINTER(C.c_int)]
         L.tapgpu_greedy.argtypes = [vp, C.c_int64, C.c_int, vp, vp, vp, vp, C.c_int,
                                     C.POINTER(_GreedyResult)]
@@ -401,6 +402,11 @@ class TapEngine:
     @property
     def lanes_delta(self):
         return lib().tapgpu_get_lanes_delta(self._h)
+
+    def set_link_passes(self, mode, window=0):
+        """Link passes: 0 by network size, 1 one workgroup per variant, 2
+        multi-block; window: step sizes per line-search window (0: 16)."""
+        self._check(lib().tapgpu_set_link_passes(self._h, int(mode), int(window)))
 
     def set_graphs(self, mode=1):
         """Replay Frank-Wolfe iterations from captured HIP graphs (1, the
