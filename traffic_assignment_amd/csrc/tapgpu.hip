@@ -1321,15 +1321,19 @@ __device__ void block_reduce_acc(tap_acc* v, tap_acc* sh /* [kBlock / kWave][kQ]
       tap_acc_merge(&v[q], o);
     }
   }
-  if (lane == 0)
+  if (lane == 0) {
+#pragma unroll
     for (int q = 0; q < kQ; ++q) sh[w * kQ + q] = v[q];
+  }
   __syncthreads();
-  if (threadIdx.x == 0)
+  if (threadIdx.x == 0) {
+#pragma unroll
     for (int q = 0; q < kQ; ++q) {
       tap_acc a = sh[q];
       for (int j = 1; j < kBlock / kWave; ++j) tap_acc_merge(&a, sh[j * kQ + q]);
       v[q] = a;
     }
+  }
 }
 
 // Partial sums per (variant, block): g0 = sum c*d, objective, vht and
@@ -1338,6 +1342,21 @@ __device__ void block_reduce_acc(tap_acc* v, tap_acc* sh /* [kBlock / kWave][kQ]
 // shuffle reduction per accumulator, with no LDS stage.
 constexpr int kLinkThreads = kWave;
 
+// acc[3 + j] += BPR(x + 2^-(jbase+j) d) * d for the kJ step sizes of a
+// window (lam0 = 2^-jbase).  Halving is exact for every step size used
+// (2^-1016 is still a normal double), so lam equals ldexp(1, -(jbase + j)).
+// Unrolled, so the sums stay in registers.
+__device__ __forceinline__ void step_sums(tap_acc* acc, double lam0, double x, double d, double t0,
+                                          double cap, double B, double pw) {
+  double lam = lam0;
+#pragma unroll
+  for (int j = 0; j < kJ; ++j) {
+    const double q = x + lam * d;
+    tap_acc_add(&acc[3 + j], tap_bpr(t0, cap, B, pw, q) * d);
+    lam *= 0.5;
+  }
+}
+
 __global__ __launch_bounds__(kLinkThreads) void k_link_partials(Links K, const VarState* st,
                                                                 tap_acc* part, int only_g) {
   const int var = blockIdx.y;
@@ -1345,9 +1364,9 @@ __global__ __launch_bounds__(kLinkThreads) void k_link_partials(Links K, const V
   if (!s.stepping || (only_g && !s.need_more)) return;
   const int jbase = only_g ? s.jnext : 0;
   tap_acc acc[kQ];
+#pragma unroll
   for (int q = 0; q < kQ; ++q) acc[q] = tap_acc{0.0, 0.0};
-  double lam[kJ];
-  for (int j = 0; j < kJ; ++j) lam[j] = ldexp(1.0, -(jbase + j));
+  const double lam0 = ldexp(1.0, -jbase);
   const size_t off = (size_t)var * K.E;
   for (int k = blockIdx.x * kLinkThreads + threadIdx.x; k < K.E; k += gridDim.x * kLinkThreads) {
     const size_t i = off + k;
@@ -1361,10 +1380,7 @@ __global__ __launch_bounds__(kLinkThreads) void k_link_partials(Links K, const V
       tap_acc_add(&acc[1], tap_integral(t0, cap, B, pw, x));
       tap_acc_add(&acc[2], x * c);
     }
-    for (int j = 0; j < kJ; ++j) {
-      const double q = x + lam[j] * d;
-      tap_acc_add(&acc[3 + j], tap_bpr(t0, cap, B, pw, q) * d);
-    }
+    step_sums(acc, lam0, x, d, t0, cap, B, pw);
   }
 #pragma unroll
   for (int q = 0; q < kQ; ++q) {
@@ -1377,6 +1393,7 @@ __global__ __launch_bounds__(kLinkThreads) void k_link_partials(Links K, const V
   }
   if (threadIdx.x == 0) {
     tap_acc* out = part + ((size_t)var * gridDim.x + blockIdx.x) * kQ;
+#pragma unroll
     for (int q = 0; q < kQ; ++q) out[q] = acc[q];
   }
 }
@@ -1388,13 +1405,14 @@ __global__ __launch_bounds__(kLinkThreads) void k_link_partials(Links K, const V
 // (thread 0 of the caller): on the first window the iteration's scalars
 // and trace record, then lambda = the first 2^-j with g <= 0, or a request
 // for the next window (need_more, jnext).
-__device__ void decide_window(VarState& s, const tap_acc* acc, int only_g, const FwParams* prm,
-                              int var) {
+__device__ __forceinline__ void decide_window(VarState& s, const tap_acc* acc, int only_g,
+                                              const FwParams* prm, int var) {
   tapgpu_iter* const trace = prm->trace;
   const int trace_cap = prm->trace_cap, max_iters = prm->max_iters;
   const double target = prm->target;
   const int jwin = prm->jwin;
   double gj[kJ];
+#pragma unroll
   for (int j = 0; j < kJ; ++j) gj[j] = tap_acc_value(acc[3 + j]);
   const int jbase = only_g ? s.jnext : 0;
   if (!only_g) {
@@ -1454,9 +1472,11 @@ __global__ __launch_bounds__(kBlock) void k_link_decide(VarState* st, const tap_
   VarState s = st[var];
   if (!s.stepping || (only_g && !s.need_more)) return;
   tap_acc acc[kQ];
+#pragma unroll
   for (int q = 0; q < kQ; ++q) acc[q] = tap_acc{0.0, 0.0};
   for (int b = threadIdx.x; b < nparts; b += kBlock) {
     const tap_acc* in = part + ((size_t)var * nparts + b) * kQ;
+#pragma unroll
     for (int q = 0; q < kQ; ++q) tap_acc_merge(&acc[q], in[q]);
   }
   __shared__ tap_acc sh[kBlock / kWave * kQ];
@@ -1505,9 +1525,9 @@ __global__ __launch_bounds__(kBlock) void k_link_small(Links K, VarState* st, un
   int jbase = 0;                     // first window, then the ones thread 0's decisions ask for
   for (int only_g = 0;; only_g = 1) {
     tap_acc acc[kQ];
+#pragma unroll
     for (int q = 0; q < kQ; ++q) acc[q] = tap_acc{0.0, 0.0};
-    double lam[kJ];
-    for (int j = 0; j < kJ; ++j) lam[j] = ldexp(1.0, -(jbase + j));
+    const double lam0 = ldexp(1.0, -jbase);
     for (int k = threadIdx.x; k < K.E; k += kBlock) {
       const size_t i = off + k;
       if (!K.present[i]) continue;
@@ -1520,10 +1540,7 @@ __global__ __launch_bounds__(kBlock) void k_link_small(Links K, VarState* st, un
         tap_acc_add(&acc[1], tap_integral(t0, cap, B, pw, x));
         tap_acc_add(&acc[2], x * c);
       }
-      for (int j = 0; j < kJ; ++j) {
-        const double q = x + lam[j] * d;
-        tap_acc_add(&acc[3 + j], tap_bpr(t0, cap, B, pw, q) * d);
-      }
+      step_sums(acc, lam0, x, d, t0, cap, B, pw);
     }
     block_reduce_acc(acc, sh);
     if (threadIdx.x == 0) {

@@ -38,6 +38,18 @@ TAP_HD double tap_pow4(double x)
   return h2 + t;
 }
 
+/* any other power: the platform pow, kept out of line so that the loops
+   over step sizes that call tap_pow stay small enough to unroll (their
+   sums then live in registers) */
+#if defined(__HIPCC__) || defined(__HIP__)
+__host__ __device__ __attribute__((noinline)) static double tap_pow_other(double x, double p)
+#else
+static double tap_pow_other(double x, double p)
+#endif
+{
+  return pow(x, p);
+}
+
 TAP_HD double tap_pow(double x, double p)
 {
   if (p == 4.0)
@@ -48,7 +60,7 @@ TAP_HD double tap_pow(double x, double p)
     return x * x;
   if (p == 0.0)
     return 1.0;
-  return pow(x, p);
+  return tap_pow_other(x, p);
 }
 
 /* BPR cost t0*(1 + B*(Q/c)^p), trunk/tap/tap_functions.c:55-65 */
