@@ -362,7 +362,42 @@ typedef struct {
   int64_t o_lo, o_hi;
   double *y;           /* private partial volumes */
   int64_t bad_orig, bad_dest;
+  const int64_t *rv_off, *rv_link;   /* in-links by head (tie mode 1), else NULL */
 } aon_job;
+
+/* 0: each node's predecessor is the link Pape's queue order leaves in
+   prevlink (the reference's choice, default); 1: on the walks, a node's
+   predecessor is re-chosen among its tight in-links u->v (u expandable,
+   u != v, fl(dist[u] + w) == dist[v], dist[u] < dist[v]) as the one with
+   the smallest dist[u], then the smallest u -- the GPU engine's rule, which
+   differs from Pape's only where several in-links are tight (exact ties).
+   A node with no such in-link keeps Pape's link. */
+static int g_tie_mode = 0;
+
+void ora_set_tie_mode(int mode) { g_tie_mode = mode; }
+
+static int64_t pred_by_rule(const aon_job *j, const double *dist, int64_t v, int64_t o,
+                            int64_t pape)
+{
+  const double dv = dist[v];
+  int64_t best = -1, best_u = 0;
+  double best_du = dv;
+  for (int64_t i = j->rv_off[v]; i < j->rv_off[v + 1]; i++) {
+    const int64_t k = j->rv_link[i];
+    const int64_t u = j->net->init[k];
+    if (u == v || (u < j->net->first_thru_node && u != o))
+      continue;
+    const double du = dist[u];
+    if (du == DBL_MAX || !(du < dv) || du + j->Wa[k] != dv)
+      continue;
+    if (best < 0 || du < best_du || (du == best_du && u < best_u)) {
+      best = k;
+      best_du = du;
+      best_u = u;
+    }
+  }
+  return best >= 0 ? best : pape;
+}
 
 /* one block of origins: Pape-LLL tree, then walk each positive-demand path
    from the destination back to the origin, restating
@@ -383,7 +418,9 @@ static void *aon_worker(void *arg)
       if (dst == o || f <= 0)
         continue;
       for (int64_t v = dst; v != o;) {
-        const int64_t k = pred[v];
+        int64_t k = pred[v];
+        if (k >= 0 && j->rv_off)
+          k = pred_by_rule(j, dist, v, o, k);
         if (k < 0) {
           j->bad_orig = o;
           j->bad_dest = dst;
@@ -410,9 +447,25 @@ int ora_fw_assign(const ora_net *net, const ora_demand *dem, const int64_t *Va,
   aon_job *jobs = calloc(nthreads, sizeof(aon_job));
   pthread_t *th = calloc(nthreads, sizeof(pthread_t));
   const int64_t per = (Z + nthreads - 1) / nthreads;   /* contiguous blocks */
+  int64_t *rv_off = NULL, *rv_link = NULL;
+  if (g_tie_mode == 1) {          /* in-links grouped by head, in link order */
+    const int64_t n = net->num_nodes + 1;
+    rv_off = calloc(n + 1, sizeof(int64_t));
+    rv_link = malloc(sizeof(int64_t) * (E > 0 ? E : 1));
+    for (int64_t k = 0; k < E; k++)
+      rv_off[net->term[k] + 1]++;
+    for (int64_t v = 0; v < n; v++)
+      rv_off[v + 1] += rv_off[v];
+    int64_t *fill = malloc(sizeof(int64_t) * n);
+    for (int64_t v = 0; v < n; v++)
+      fill[v] = rv_off[v];
+    for (int64_t k = 0; k < E; k++)
+      rv_link[fill[net->term[k]]++] = k;
+    free(fill);
+  }
   for (int t = 0; t < nthreads; t++) {
     jobs[t] = (aon_job){net, dem, Va, Ea, Wa, 1 + t * per, t * per + per,
-                        calloc(E, sizeof(double)), 0, 0};
+                        calloc(E, sizeof(double)), 0, 0, rv_off, rv_link};
     if (jobs[t].o_hi > Z) jobs[t].o_hi = Z;
     if (nthreads == 1)
       aon_worker(&jobs[t]);
@@ -436,6 +489,7 @@ int ora_fw_assign(const ora_net *net, const ora_demand *dem, const int64_t *Va,
     free(jobs[t].y);
   }
   free(jobs); free(th);
+  free(rv_off); free(rv_link);
   return rc;
 }
 

@@ -20,7 +20,10 @@
  *       Dijkstra on the reference's rmat.txt.* graphs, fixed-point invariants.
  *   ora_bpr, ora_integral, ora_line_search -- closed forms, hand cases,
  *       golden lambda sequences.
- *   ora_fw_assign -- brute-force path sums; golden flows.
+ *   ora_fw_assign -- brute-force path sums; golden flows.  Tie mode 1
+ *       (ora_set_tie_mode) against a brute-force loading by the tie rule on
+ *       networks with thousands of exact ties, and equal to mode 0 where
+ *       shortest paths are unique.
  *   ora_fw_solve  -- golden per-iteration scalars and final flows from the
  *       compiled reference (tap_frankwolfe_pthread.c built from source).
  */
@@ -70,6 +73,12 @@ void ora_sssp_all(const int64_t *Va, const int64_t *Ea, const double *Wa,
    rounded (to compare the GPU engine bit for bit) */
 void ora_set_pow_mode(int mode);
 double ora_pow4_exact_check(double x);
+
+/* predecessors on the AON walks: 0 = Pape's prevlink (the reference's
+   choice, default); 1 = the GPU engine's tie rule (among tight in-links
+   with a smaller tail label: smallest tail label, then smallest tail id),
+   which differs from Pape's only at exact ties */
+void ora_set_tie_mode(int mode);
 
 double ora_bpr(double t0, double cap, double B, double power, double Q);
 double ora_integral(double t0, double cap, double B, double power, double Q);

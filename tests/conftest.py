@@ -125,7 +125,8 @@ def random_network(seed, cost_kind):
     cost_kind "ints"), non-through zones, start nodes that are not zones,
     costs over twelve orders of magnitude ("wide").  Through nodes are
     joined by a two-way ring and every non-through node by a two-way link to
-    one through node, so every zone reaches every other."""
+    one through node, so every zone reaches every other.  "pints": costs
+    1, 2 or 3 (thousands of exact ties, every link costs something)."""
     from traffic_assignment_amd import synth
     rng = np.random.default_rng(seed)
     N = int(rng.integers(40, 2500))
@@ -155,6 +156,8 @@ def random_network(seed, cost_kind):
     E = init.size
     if cost_kind == "ints":
         cost = rng.integers(0, 4, E).astype(np.float64)
+    elif cost_kind == "pints":           # exact ties, but no zero-cost links
+        cost = rng.integers(1, 4, E).astype(np.float64)
     elif cost_kind == "wide":
         cost = 10.0 ** rng.uniform(-6, 6, E)
     else:

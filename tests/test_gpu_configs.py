@@ -6,6 +6,10 @@
 * the batched workload: 512 budgeted upgrade subsets of metro (32 candidate
   upgrades) in one context, to relgap 1e-4: every variant converges; sampled
   variants equal their plain solves and the oracle bit for bit.
+* regional (94,000 nodes / 386,800 links / 4,000 zones) solved to relgap 1e-4
+  on the labels-in-HBM path, bit-identical to the oracle with the engine's
+  tie rule; against Pape's tie choice, the same step sizes and a few links
+  apart.
 * equilibrium gradients on the regional network: at a quarter of its size
   against the CPU path-space gradient and central differences of re-solved
   equilibria on 64 random links (step 1e-4 c, 1e-4 relative); at full size
@@ -98,6 +102,46 @@ def test_batched_workload_512_variants(engine_mod, oracle_lib):
         assert o["iterations"] == pr["iterations"], v
         assert np.array_equal(o["trace"]["lambda_"], ptr["lambda_"]), v
         assert np.array_equal(o["x"], px) and np.array_equal(o["c"], pc), v
+
+
+def test_regional_solve_to_target_bit_identical(engine_mod, oracle_lib):
+    """The regional network (94,000 nodes / 386,800 links / 4,000 zones,
+    gravity demand on free-flow times) solved to relgap 1e-4 on the
+    labels-in-HBM path.
+
+    Some later iterations meet exact ties (two shortest paths of equal
+    cost), where Pape's queue order and the engine's predecessor rule load
+    different, equally short paths.  Against the oracle with the engine's
+    rule (tie mode 1): every step size, volume and cost bit-identical.
+    Against the oracle with Pape's choice (the reference's): the same
+    iterations and step sizes, the relative gap within 1e-9, and volumes
+    that differ on a few links only (12 of 386,800 on this network)."""
+    print("case regional full solve", flush=True)
+    net, dem = network("regional")
+    with engine_mod.TapEngine(net, dem) as eng:
+        eng.set_strategy("hbm")
+        eng.fw_init()
+        r = eng.fw_run(target_relgap=TARGET)[0]
+        x, c = eng.flows()
+        tr = eng.trace()
+    print("iterations", r["iterations"], flush=True)
+    lib = oracle_lib.lib()
+    lib.ora_set_tie_mode(1)
+    try:
+        o1 = _oracle_solve(oracle_lib, net, dem, target_relgap=TARGET)
+    finally:
+        lib.ora_set_tie_mode(0)
+    assert r["converged"] and r["iterations"] == o1["iterations"] > 10
+    assert np.array_equal(tr["lambda_"], o1["trace"]["lambda_"])
+    assert np.array_equal(x, o1["x"]) and np.array_equal(c, o1["c"])
+    assert abs(tr["relgap"][-1] - o1["relgap"]) <= 1e-9 * o1["relgap"]
+    o0 = _oracle_solve(oracle_lib, net, dem, target_relgap=TARGET)
+    assert o0["iterations"] == r["iterations"]
+    assert np.array_equal(tr["lambda_"], o0["trace"]["lambda_"])
+    assert abs(tr["relgap"][-1] - o0["relgap"]) <= 1e-9 * o0["relgap"]
+    differ = int(np.sum(x != o0["x"]))
+    print("links differing from the Pape-order solve:", differ, flush=True)
+    assert differ <= 1e-4 * net.num_links
 
 
 def test_regional_quarter_gradient_against_reference_and_fd(engine_mod):

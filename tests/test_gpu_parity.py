@@ -101,9 +101,32 @@ def test_sssp_random_networks(engine_mod, oracle_lib, seed, cost_kind, strategy)
     print("equal-cost ties:", ties)
 
 
+def _shortest_path_loading_identities(oracle_lib, net, dem, y):
+    """Exact checks that hold for every shortest-path loading (integers
+    throughout): total cost sum y*c equals sum over OD of demand x label,
+    and flow balances at every node."""
+    Z = net.num_zones
+    orig = np.repeat(np.arange(1, Z + 1), np.diff(dem.start))
+    keep = (dem.dest != orig) & (dem.demand > 0)
+    tot = 0.0
+    for o in range(1, Z + 1):
+        m = keep & (orig == o)
+        if m.any():
+            d, _ = oracle_lib.sssp(net, net.fftime, o)
+            tot += float(np.sum(dem.demand[m] * d[dem.dest[m]]))
+    assert float(np.sum(y * net.fftime)) == tot
+    bal = np.zeros(net.num_nodes + 1)
+    np.add.at(bal, net.term, y)
+    np.add.at(bal, net.init, -y)
+    want = np.zeros(net.num_nodes + 1)
+    np.add.at(want, dem.dest[keep], dem.demand[keep])
+    np.add.at(want, orig[keep], -dem.demand[keep])
+    assert np.array_equal(bal, want)
+
+
 @pytest.mark.parametrize("strategy", LOAD_STRATS)
 @pytest.mark.parametrize("seed,cost_kind", [(7, "cont"), (8, "wide"), (9, "cont"), (12, "wide"),
-                                            (13, "ints")])
+                                            (13, "ints"), (14, "pints"), (15, "pints")])
 def test_aon_random_networks(engine_mod, oracle_lib, seed, cost_kind, strategy):
     """Integer demand between random zone pairs on random networks: volumes
     exactly the oracle's."""
@@ -116,32 +139,28 @@ def test_aon_random_networks(engine_mod, oracle_lib, seed, cost_kind, strategy):
     if cost_kind == "ints":
         # small-integer costs, zero-cost cycles: thousands of exact ties,
         # which Pape's queue order and the engine's rules break differently,
-        # so the trees (and volumes) may differ from the oracle's.  Exact
-        # checks that hold for every shortest-path loading (integers
-        # throughout): total cost sum y*c equals sum over OD of demand x
-        # label, and flow balances at every node.  The per-origin kernels and
-        # the lanes kernel (which hands level ties to them) share one rule:
-        # bit-identical volumes.
-        Z = net.num_zones
-        orig = np.repeat(np.arange(1, Z + 1), np.diff(dem.start))
-        keep = (dem.dest != orig) & (dem.demand > 0)
-        tot = 0.0
-        for o in range(1, Z + 1):
-            m = keep & (orig == o)
-            if m.any():
-                d, _ = oracle_lib.sssp(net, net.fftime, o)
-                tot += float(np.sum(dem.demand[m] * d[dem.dest[m]]))
-        assert float(np.sum(y * net.fftime)) == tot
-        bal = np.zeros(net.num_nodes + 1)
-        np.add.at(bal, net.term, y)
-        np.add.at(bal, net.init, -y)
-        want = np.zeros(net.num_nodes + 1)
-        np.add.at(want, dem.dest[keep], dem.demand[keep])
-        np.add.at(want, orig[keep], -dem.demand[keep])
-        assert np.array_equal(bal, want)
+        # so the trees (and volumes) may differ from the oracle's; the
+        # identities of any shortest-path loading hold.  The per-origin
+        # kernels and the lanes kernel (which hands level ties to them)
+        # share one rule: bit-identical volumes.
+        _shortest_path_loading_identities(oracle_lib, net, dem, y)
         if strategy != "sweeps":
             with make_engine(engine_mod, net, dem, "hbm") as ref:
                 assert np.array_equal(y, ref.aon())
+    elif cost_kind == "pints":
+        # costs 1, 2, 3: thousands of exact ties, no zero-cost links.  The
+        # per-origin and lanes kernels pick by the engine's rule, which the
+        # oracle's tie mode 1 restates: bit-identical volumes.  The node
+        # sweeps pick during their rounds; the identities hold.
+        _shortest_path_loading_identities(oracle_lib, net, dem, y)
+        if strategy != "sweeps":
+            lib = oracle_lib.lib()
+            lib.ora_set_tie_mode(1)
+            try:
+                o1 = oracle_lib.fw_assign(net, dem, net.fftime, nthreads=8)
+            finally:
+                lib.ora_set_tie_mode(0)
+            assert np.array_equal(y, o1)
     else:
         assert np.array_equal(y, oracle_lib.fw_assign(net, dem, net.fftime, nthreads=8))
     if strategy == "lanes":
@@ -150,7 +169,7 @@ def test_aon_random_networks(engine_mod, oracle_lib, seed, cost_kind, strategy):
         # per-origin kernel redoes
         print("lanes launches", st["lanes_launches"], "fallbacks", st["lanes_fallbacks"])
         assert st["lanes_launches"] + st["lanes_fallbacks"] == 1
-        if cost_kind == "wide":
+        if cost_kind in ("wide", "pints"):
             assert st["lanes_fallbacks"] == 0
 
 

@@ -8,7 +8,7 @@ from fractions import Fraction
 import numpy as np
 import pytest
 
-from conftest import network, pred_reaches_origin, random_network, rmat
+from conftest import network, pred_reaches_origin, random_demand, random_network, rmat
 
 
 # --- link functions ----------------------------------------------------------
@@ -230,3 +230,76 @@ def test_aon_thread_count_does_not_change_integer_volumes(oracle_lib):
     a = oracle_lib.fw_assign(net, dem, net.fftime, nthreads=1)
     b = oracle_lib.fw_assign(net, dem, net.fftime, nthreads=7)
     assert np.array_equal(a, b)
+
+
+def _aon_by_rule(oracle_lib, net, dem, costs, pick_max_tail=False):
+    """AON volumes with each walked node's predecessor chosen by the tie
+    rule, from the oracle's labels: among in-links u->v with u expandable,
+    u != v, dist[u] < dist[v] and dist[u] + c == dist[v], the smallest
+    dist[u], then the smallest u (the largest u with pick_max_tail, a
+    planted mistake)."""
+    u, v = net.init, net.term
+    ft = net.first_thru_node
+    y = np.zeros(net.num_links)
+    ties = 0
+    for o in range(1, dem.num_zones + 1):
+        items = range(dem.start[o - 1], dem.start[o])
+        if not any(dem.dest[i] != o and dem.demand[i] > 0 for i in items):
+            continue
+        d, p = oracle_lib.sssp(net, costs, o)
+        big = np.finfo(float).max
+        tight = (((u >= ft) | (u == o)) & (u != v) & (d[u] < big) & (d[u] < d[v])
+                 & (d[u] + costs == d[v]))
+        k = np.nonzero(tight)[0]
+        ties += int(np.sum(np.bincount(v[k], minlength=net.num_nodes + 1) >= 2))
+        tail_key = -u[k] if pick_max_tail else u[k]
+        k = k[np.lexsort((tail_key, d[u[k]], v[k]))]
+        first = np.r_[True, v[k][1:] != v[k][:-1]]
+        rule = p.copy()
+        rule[v[k][first]] = k[first]
+        for i in items:
+            dst, f = int(dem.dest[i]), float(dem.demand[i])
+            if dst == o or f <= 0:
+                continue
+            w = dst
+            while w != o:
+                e = rule[w]
+                y[e] += f
+                w = u[e]
+    return y, ties
+
+
+@pytest.mark.parametrize("seed", [21, 22, 23])
+def test_aon_tie_rule_mode_brute_force(oracle_lib, seed):
+    """Tie mode 1 (the engine's predecessor rule) against a brute-force
+    loading by the rule on random networks with costs 1, 2, 3 (thousands of
+    exact ties, no zero-cost links).  The ties are real (counted), and the
+    planted mistake (largest tail id first) changes the volumes."""
+    net, rng = random_network(seed, "pints")
+    dem = random_demand(net, rng)
+    lib = oracle_lib.lib()
+    lib.ora_set_tie_mode(1)
+    try:
+        y1 = oracle_lib.fw_assign(net, dem, net.fftime, nthreads=4)
+    finally:
+        lib.ora_set_tie_mode(0)
+    y0 = oracle_lib.fw_assign(net, dem, net.fftime, nthreads=4)
+    ref, ties = _aon_by_rule(oracle_lib, net, dem, net.fftime)
+    assert ties > 100
+    assert np.array_equal(y1, ref)
+    assert not np.array_equal(y1, y0)          # Pape's queue order breaks ties differently
+    bad, _ = _aon_by_rule(oracle_lib, net, dem, net.fftime, pick_max_tail=True)
+    assert not np.array_equal(bad, ref)
+
+
+def test_aon_tie_rule_mode_without_ties(oracle_lib):
+    """Where every shortest path is unique, tie mode 1 loads exactly as
+    Pape's predecessors do."""
+    net, dem = network("metro", 0.3)
+    lib = oracle_lib.lib()
+    lib.ora_set_tie_mode(1)
+    try:
+        y1 = oracle_lib.fw_assign(net, dem, net.fftime, nthreads=4)
+    finally:
+        lib.ora_set_tie_mode(0)
+    assert np.array_equal(y1, oracle_lib.fw_assign(net, dem, net.fftime, nthreads=4))
