@@ -251,6 +251,34 @@ def test_load_pred_rejects_non_trees(engine_mod, oracle_lib):
         assert np.array_equal(eng.load_pred(op), y)
 
 
+def test_negative_costs_rejected(engine_mod):
+    """The kernels order labels by their bit patterns, which holds for
+    non-negative costs: a negative (or NaN) free-flow time or B at creation,
+    a negative cost in tapgpu_set_costs and a negative free-flow time in
+    tapgpu_set_links are errors, and a rejected call changes nothing."""
+    import dataclasses
+    net, dem = network("grid5")
+    t0 = net.fftime.copy()
+    t0[3] = -1.0
+    with pytest.raises(engine_mod.TapGpuError, match="negative or NaN"):
+        engine_mod.TapEngine(dataclasses.replace(net, fftime=t0), dem)
+    B = net.B.copy()
+    B[5] = np.nan
+    with pytest.raises(engine_mod.TapGpuError, match="negative or NaN"):
+        engine_mod.TapEngine(dataclasses.replace(net, B=B), dem)
+    with engine_mod.TapEngine(net, dem) as eng:
+        eng.fw_init()
+        eng.fw_run(max_iters=3)
+        x0, c0 = eng.flows()
+        with pytest.raises(engine_mod.TapGpuError, match="negative or NaN cost"):
+            eng.set_costs(np.where(np.arange(net.num_links) == 2, -0.5, net.fftime))
+        with pytest.raises(engine_mod.TapGpuError, match="negative or NaN free-flow"):
+            eng.set_links(capacity=net.capacity * 2, fftime=t0)
+        x1, c1 = eng.flows()
+        assert np.array_equal(x0, x1) and np.array_equal(c0, c1)
+        eng.fw_run(max_iters=4)         # the context still runs
+
+
 def test_single_source_sssp_entry_points(engine_mod, oracle_lib, hostlib):
     """sssp_pape / sssp_pape_lll / sssp_slf / sssp_slf_lll /
     sssp_prevnodefirst (trunk/tap/sssp_pape.h:36-64) through the C ABI:

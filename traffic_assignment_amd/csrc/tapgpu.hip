@@ -2736,6 +2736,11 @@ int build(tapgpu_ctx* c, const tapgpu_network* net, const tapgpu_demand* dem, in
       hpw[j] = pw_v ? pw_v[j] : net->power[k];
       if (present) hpres[j] = present[j] ? 1 : 0;
       if (hpres[j] && hcap[j] > 0.0 && hcap[j] < minpos) minpos = hcap[j];
+      // the kernels order labels by their bit patterns, which holds for
+      // non-negative costs only: BPR costs are non-negative when t0 and B are
+      if (hpres[j] && !(ht0[j] >= 0.0 && hB[j] >= 0.0))
+        return c->fail(TAPGPU_E_ARG, "link %lld of variant %lld: negative or NaN free-flow time or B",
+                       (long long)k, (long long)v);
     }
     for (int64_t k = 0; k < E; ++k) {
       const size_t j = (size_t)v * E + k;
@@ -2895,6 +2900,9 @@ int tapgpu_set_costs(tapgpu_ctx* c, int64_t variant, const double* costs) {
   if (!c || !costs) return TAPGPU_E_ARG;
   int rc = check_variant(c, variant);
   if (rc) return rc;
+  for (int k = 0; k < c->E; ++k)
+    if (!(costs[k] >= 0.0))
+      return c->fail(TAPGPU_E_ARG, "link %d: negative or NaN cost", k);
   if ((rc = upload(c, c->c + (size_t)variant * c->E, costs, c->E))) return rc;
   if ((rc = launch_costs(c, 2, nullptr, (int)variant, 1))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -3634,6 +3642,10 @@ int tapgpu_set_links(tapgpu_ctx* c, int64_t variant, const double* capacity,
   if (!c) return TAPGPU_E_ARG;
   int rc = check_variant(c, variant);
   if (rc) return rc;
+  if (free_flow_time)       // (costs must stay non-negative; checked before any change)
+    for (int k = 0; k < c->E; ++k)
+      if (!(free_flow_time[k] >= 0.0))
+        return c->fail(TAPGPU_E_ARG, "link %d: negative or NaN free-flow time", k);
   const size_t off = (size_t)variant * c->E;
   if (capacity) {
     std::vector<unsigned char> pres(c->E);
