@@ -62,7 +62,8 @@ def origin_bytes(st, num_links) -> float:
     """Algorithmic bytes of the one-origin-per-workgroup kernel, summed over
     its (origin, variant) workgroups: the same worklist formula with one
     origin per row (8 B per expansion + its label, 12 B of link data and a
-    label per relaxed link, 12 B per lowered label), plus predecessor
+    label per relaxed link, 12 B per lowered label -- for the labels-in-HBM
+    kernel, per atomic min issued), plus predecessor
     selection (per link: 16 B of reverse-CSR data and the tail's label) and
     loading (per walk step 2 B of tree data and 8 B of flow
     read-modify-write; per tree link with flow 8 B of flow, 2 B of tree data
@@ -168,7 +169,7 @@ def lanes_bytes(st, num_nodes, in_degree) -> float:
     group of 64 variants), from its counters.  Relaxation, per wavefront
     expansion (a node in all 64 lanes): its label row 512 B; per out-link
     scanned: 4 B head, the cost row 512 B and the head's label row 512 B; per
-    lane lowered 8 B.  Label rows set at the start of each task: 512 B per
+    lane atomic min (a candidate below the label read) 8 B.  Label rows set at the start of each task: 512 B per
     node.  Loading, per lane and walk step: 8 B memo, 4 B tail, 16 B volume
     read-modify-write; per predecessor selection (lane): 8 B own label and
     8 B memo, and per in-link 8 B reverse-CSR data, 8 B cost, 8 B tail label

@@ -220,7 +220,9 @@ typedef struct {
   int64_t origin_launches, origin_blocks;
   int64_t origin_expansions;   /* node expansions (label-correcting)        */
   int64_t origin_scanned;      /* out-links relaxed                         */
-  int64_t origin_updates;      /* labels lowered                            */
+  int64_t origin_updates;      /* labels lowered (labels-in-HBM and lanes   */
+                               /* kernels: atomic mins issued, one per      */
+                               /* candidate below the label read)           */
   int64_t origin_relax_rounds;  /* relaxation rounds, summed over origins    */
   int64_t origin_load_rounds;   /* longest loading walk, summed over origins */
   int64_t origin_loaded;        /* tree links given flow                     */

@@ -1069,12 +1069,13 @@ __global__ __launch_bounds__(kHbmBlock, kHbmPerCu * kHbmBlock / kWave / 4) void 
           const unsigned long long cb = as_b(du + hw[k]);
           bool low = false;
           if (cb < hd[k]) {
-            const unsigned long long old =
-                __hip_atomic_fetch_min(&dist[v], cb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (cb < old) {
-              ++n_upd;
-              low = v >= A.ft0;
-            }
+            // no wait for the old value: a head whose label this lowers is
+            // listed (a node listed twice is deduplicated by the bitmap, and
+            // its key -- cb, at or above the label -- is re-read before a
+            // bucket decision), so the atomic need not return
+            (void)__hip_atomic_fetch_min(&dist[v], cb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            ++n_upd;
+            low = v >= A.ft0;
           }
           list(low, v, cb);
         }
