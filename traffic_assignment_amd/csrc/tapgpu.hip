@@ -3311,7 +3311,9 @@ static int capture(tapgpu_ctx* c, hipGraphExec_t* exec, F&& issue) {
     *exec = nullptr;
   }
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;               // (launch paths then skip host-side set-up)
   const int rc = issue();
+  c->capturing = false;
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (rc) {
@@ -3381,7 +3383,6 @@ static int lanes_graph(tapgpu_ctx* c, const Links& K, dim3 lgrid, int G) {
   const long long key[6] = {3 + (c->origin_order_ready ? 64 : 0) + 128 * c->link_mode, G, c->lanes_qcap,
                             dbits, (long long)c->lanes_lds, c->lanes_grid};
   if (c->g_iter && std::equal(key, key + 6, c->g_iter_key)) return TAPGPU_OK;
-  c->capturing = true;
   int rc = capture(c, &c->g_iter, [&]() {
     hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(kBlock), 0, c->stream, c->st, c->V, c->var_on,
                        c->err_dev, c->counters, c->y, 0ll, 0);
@@ -3390,7 +3391,6 @@ static int lanes_graph(tapgpu_ctx* c, const Links& K, dim3 lgrid, int G) {
     hipLaunchKernelGGL(k_lanes_gate, dim3(1), dim3(kBlock), 0, c->stream, c->err_dev, c->st, c->V);
     return issue_link_passes(c, K, lgrid);
   });
-  c->capturing = false;
   if (rc) return rc;
   std::copy(key, key + 6, c->g_iter_key);
   return TAPGPU_OK;
