@@ -64,6 +64,51 @@ def test_batched_equals_plain_and_oracle(engine_mod, oracle_lib, strategy):
             assert np.array_equal(o["x"], px)
 
 
+def test_lanes_exact_ties_many_variants(engine_mod, oracle_lib):
+    """70 variants (two lane groups) of a random network whose free-flow
+    times are 1, 2 or 3 per variant: the free-flow loading meets thousands
+    of exact ties in every lane.  The lanes kernel (each lane walks its own
+    tree by the tie rule) equals the per-origin kernel variant by variant
+    over fw_init and 3 iterations, and sampled variants equal the oracle
+    with the engine's tie rule (tie mode 1), bit for bit."""
+    from conftest import random_demand, random_network
+    print("case lanes exact ties, 70 variants", flush=True)
+    net, rng = random_network(31, "pints")
+    dem = random_demand(net, rng)
+    V = 70
+    t0 = rng.integers(1, 4, (V, net.num_links)).astype(np.float64)
+    out = {}
+    for path in ("lanes", "hbm"):
+        with engine_mod.TapEngine(net, dem, num_variants=V, fftime=t0) as eng:
+            if path == "lanes":
+                eng.set_lanes(1)
+            else:
+                eng.set_lanes(0)
+                eng.set_strategy("hbm")
+            eng.fw_init()
+            eng.fw_run(max_iters=3)
+            st = eng.stats()
+            out[path] = [eng.flows(v) for v in range(V)]
+        if path == "lanes":
+            assert st["lanes_launches"] > 0 and st["lanes_fallbacks"] == 0
+    for v in range(V):
+        assert np.array_equal(out["lanes"][v][0], out["hbm"][v][0]), v
+        assert np.array_equal(out["lanes"][v][1], out["hbm"][v][1]), v
+    lib = oracle_lib.lib()
+    lib.ora_set_pow_mode(1)
+    lib.ora_set_tie_mode(1)
+    try:
+        for v in (0, 41, 69):
+            import dataclasses
+            nv = dataclasses.replace(net, fftime=t0[v].copy())
+            o = oracle_lib.fw_solve(nv, dem, max_iters=3, nthreads=8)
+            assert np.array_equal(o["x"], out["lanes"][v][0]), v
+            assert np.array_equal(o["c"], out["lanes"][v][1]), v
+    finally:
+        lib.ora_set_tie_mode(0)
+        lib.ora_set_pow_mode(0)
+
+
 def test_batched_variants_stop_independently(engine_mod):
     base, dem, mods, subsets = _small()
     eng, u, present = variants.make_engine(base, dem, mods, subsets)
