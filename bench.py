@@ -153,15 +153,23 @@ def cpu_baselines(net, dem, gpu_iters: int):
     return allc
 
 
-def load_traffic(workload):
+def load_traffic(workload, kernel):
+    """Calibrated HBM bytes per launch of `kernel` on `workload` from the
+    rocprof counter runs (profiles/traffic.json), or None when that kernel
+    was not measured there."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
         return None
     try:
         with open(p) as f:
-            return json.load(f).get(workload, {}).get("hbm_bytes_per_launch")
+            d = json.load(f)
     except Exception:
         return None
+    for key in (workload, f"{workload}:{kernel}"):
+        e = d.get(key, {})
+        if e.get("kernel") == kernel:
+            return e.get("hbm_bytes_per_launch")
+    return None
 
 
 def lanes_bytes(st, num_nodes, in_degree) -> float:
@@ -181,9 +189,10 @@ def lanes_bytes(st, num_nodes, in_degree) -> float:
     return relax + init + load
 
 
-def loading_roofline(acc, path, num_nodes, num_links, traffic=None):
+def loading_roofline(acc, path, num_nodes, num_links, workload=None):
     """Roofline of the loading kernel from accumulated stats (HIP events
-    around each of its launches)."""
+    around each of its launches); traffic from the counter runs of that
+    kernel on `workload`."""
     launches = acc.get("origin_launches", 0)
     avg_ms = acc.get("origin_kernel_ms", 0.0) / max(launches, 1)
     if acc.get("lanes_launches", 0) and acc.get("lanes_launches") == launches:
@@ -196,7 +205,8 @@ def loading_roofline(acc, path, num_nodes, num_links, traffic=None):
     achieved = bpl / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     return {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2),
             "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5),
-            "traffic": traffic, "avg_launch_us": round(avg_ms * 1e3, 3),
+            "traffic": load_traffic(workload, kernel) if workload else None,
+            "avg_launch_us": round(avg_ms * 1e3, 3),
             "launches": int(launches), "algorithmic_bytes_per_launch": round(bpl, 1)}
 
 
@@ -235,8 +245,7 @@ def regional_sssp(args):
         eng.fw_run(max_iters=1 + args.regional_iters)
         st = eng.stats()
         path = eng.strategy
-    roof = loading_roofline(st, path, net.num_nodes, net.num_links,
-                            load_traffic("regional"))
+    roof = loading_roofline(st, path, net.num_nodes, net.num_links, "regional")
     roof["workload"] = DESCR["regional"]
     roof["iterations_timed"] = args.regional_iters
     roof["delta_mean_t0"] = st.get("origin_delta", 0.0)
@@ -333,7 +342,7 @@ def bench_batched(args):
     eng.set_profiling(False)
     path = eng.strategy
     lanes_min = eng.lanes
-    roof = loading_roofline(acc, path, u.num_nodes, u.num_links, load_traffic("batched"))
+    roof = loading_roofline(acc, path, u.num_nodes, u.num_links, "batched")
     if args.no_solve:           # profiling runs: the K steps only
         eng.close()
         print(json.dumps({"metric": METRIC, "value": None, "unit": "solves/s", "n_gpus": 1,
@@ -584,7 +593,7 @@ def main():
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     roof = {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2),
             "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5),
-            "traffic": load_traffic(args.workload),
+            "traffic": load_traffic(args.workload, kernel),
             "avg_launch_us": round(avg_ms * 1e3, 3), "launches": int(launches),
             "algorithmic_bytes_per_launch": round(bytes_per_launch, 1)}
     cpu = None
