@@ -296,9 +296,9 @@ int tapgpu_get_origin_block(const tapgpu_ctx *ctx);
 /* Worklist schedule of the labels-in-HBM path (strategy 3): bucket width of
    its label-ordered rounds, as a multiple of the mean free-flow link time
    (0: no buckets, every queued node is expanded each round).  -1 (the
-   default): tuned on the first loading by timing each candidate
-   (0, 1, 2, 4, 8, 16) on one wave of workgroups.  Results do not depend
-   on it. */
+   default): tuned on the first loading: each candidate (0, 1, 2, 4, 8,
+   16) runs one wave of workgroups, and the smallest busy time summed over
+   the workgroups wins.  Results do not depend on it. */
 int tapgpu_set_origin_delta(tapgpu_ctx *ctx, double mult);
 double tapgpu_get_origin_delta(const tapgpu_ctx *ctx);
 
@@ -326,8 +326,9 @@ int tapgpu_get_lanes(const tapgpu_ctx *ctx);
 /* Worklist schedule of the lanes kernel: bucket width of its label-ordered
    rounds (on the smallest label over a node's lanes), as a multiple of the
    mean free-flow link time (0: no buckets).  -1 (the default): tuned on the
-   first loading by timing each candidate (0, 1, 2, 4, 8) on one wave of
-   workgroups.  Results do not depend on it. */
+   first loading: each candidate (0, 1, 2, 4, 8) runs one wave of
+   workgroups, and the smallest busy time summed over them wins.  Results
+   do not depend on it. */
 int tapgpu_set_lanes_delta(tapgpu_ctx *ctx, double mult);
 double tapgpu_get_lanes_delta(const tapgpu_ctx *ctx);
 

@@ -2358,9 +2358,10 @@ int ensure_hbm(tapgpu_ctx* c) {
 // kernel (persistent grid over the Z * Vn (origin, variant) items).
 //
 // Bucket-width tuning: the first loading on a context with delta_mult < 0
-// times one wave of workgroups (the first hbm_grid items) for each
-// candidate width, in units of the mean free-flow link time, and keeps the
-// fastest; the full launch follows.  Results do not depend on the width.
+// runs one wave of workgroups (the first hbm_grid items) for each candidate
+// width, in units of the mean free-flow link time, and keeps the one with
+// the smallest busy time summed over the workgroups; the full launch
+// follows.  Results do not depend on the width.
 constexpr double kDeltaCandidates[] = {0.0, 1.0, 2.0, 4.0, 8.0, 16.0};
 
 int launch_hbm(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on, int items_cap,
@@ -2398,25 +2399,40 @@ int launch_hbm(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on, int i
   return TAPGPU_OK;
 }
 
+// Busy time of a tuning launch: the workgroups' phase clocks (relaxation,
+// predecessors, loading), summed over its tasks.  At full occupancy the
+// smallest sum is the highest throughput; one wave's wall time would be set
+// by its slowest task alone.
+int tuning_work(tapgpu_ctx* c, double* work, int* herr) {
+  unsigned long long hc[16];
+  HIPCHK(c, hipMemcpyAsync(hc, c->counters, sizeof hc, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(herr, c->err_dev, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *work = (double)hc[8] + (double)hc[9] + (double)hc[10];
+  return TAPGPU_OK;
+}
+
 int tune_delta(tapgpu_ctx* c, int v0, int Vn, const unsigned char* var_on) {
   if (!c->ev[0])
     for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
   float best_ms = 0.f;
-  double best = 4.0;
+  double best = 4.0, best_work = 0.0;
   for (double m : kDeltaCandidates) {
     c->delta_mult = m;
+    HIPCHK(c, hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 16, c->stream));
     float ms = 0.f;
     int rc = launch_hbm(c, v0, Vn, var_on, c->hbm_grid, &ms);
     if (rc) return rc;
     int herr = 0;
-    HIPCHK(c, hipMemcpyAsync(&herr, c->err_dev, sizeof herr, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double work = 0.0;
+    if ((rc = tuning_work(c, &work, &herr))) return rc;
     if (herr) break;
-    if (best_ms == 0.f || ms < best_ms) { best_ms = ms; best = m; }
+    if (best_work == 0.0 || work < best_work) { best_work = work; best_ms = ms; best = m; }
   }
   c->delta_mult = best;
   c->stats.origin_tune_ms += best_ms;
   HIPCHK(c, hipMemsetAsync(c->err_dev, 0, sizeof(int) * 3, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 16, c->stream));
   return TAPGPU_OK;
 }
 
