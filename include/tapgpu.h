@@ -337,9 +337,9 @@ double tapgpu_get_lanes_delta(const tapgpu_ctx *ctx);
    loading's schedule is tuned, each iteration (opening kernel, loading,
    both line-search windows, step, readback) is a replay of a captured HIP
    graph -- on the per-origin LDS and HBM paths and on the lanes kernel;
-   elsewhere the link passes are.  On the per-origin paths an iteration
-   shorter than 0.5 ms goes 8 to a replay (2 below 2 ms), so the launch and
-   the synchronisation are paid once for several.  mode = 2..8: that many
+   elsewhere the link passes are.  On the per-origin paths iterations
+   shorter than 2 ms go 8 to a replay, so the launch and the
+   synchronisation are paid once for several.  mode = 2..8: that many
    iterations per replay on those paths.  mode = 0: the same launches issued
    one by one.  Profiling (tapgpu_set_profiling) issues them one by one too,
    so the events around the loading are recorded.  Results do not depend on
