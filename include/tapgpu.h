@@ -87,7 +87,8 @@ int tapgpu_create_batched(tapgpu_ctx **ctx, const tapgpu_network *net,
 
 void tapgpu_destroy(tapgpu_ctx *ctx);
 
-/* last error message of this context (or of the last failed create) */
+/* last error message of this context (NULL: of the calling thread's last
+   failed create) */
 const char *tapgpu_last_error(const tapgpu_ctx *ctx);
 
 int64_t tapgpu_num_variants(const tapgpu_ctx *ctx);

@@ -1788,7 +1788,7 @@ struct tapgpu_ctx {
 
 namespace {
 
-std::string g_create_error;
+thread_local std::string g_create_error;   // message of this thread's last failed create
 
 #define HIPCHK(ctx, call)                                                               \
   do {                                                                                  \
